@@ -1,0 +1,141 @@
+"""Benchmark of the split-step path on one MI355X.
+
+Workload (one "step" = one round trip): the singly-resonant cavity solve,
+64x64x256 complex64 fields, MgO:PPLN 5 mm, NZ = 100 (100 slices per trip),
+Rs = 0.98, idler not resonant, GDD compensation, free-space feedback; the
+fields start from a focused 2 ps pump and seeded noise and stay resident in
+HBM for the whole timed region.  value = cell-steps/s (cells x slices x
+round trips / wall time).
+
+The roofline entry is for the dominant kernel, timed with HIP events on the
+solve's stream (opo_profile_trips) over a second run of the same trips; its
+algorithmic bytes are 16 B per cell per field touched (8 read + 8 written).
+The CPU baseline is the FP64 oracle on the host cores over the first round
+trips of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "cell-steps/s & %HBM roofline; p50/p99 latency of batched OPO sweeps"
+HBM_PEAK_GBS = 8000.0
+# fields touched per launch of each stage kind
+STAGE_FIELDS = {"nonlinear": 3, "time_op": 3, "xy_op": 3, "free_space": 2, "gdd": 2, "power": 3}
+STAGE_RW = {"power": 1}  # the power reduction only reads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-trips", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-trips", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one solve per GPU on one GPU; --gpus {args.gpus} is not "
+              "supported (independent solves shard across GPUs by footprint, no collectives).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cuopo3d_amd as co
+    from cuopo3d_amd import configs
+    from cuopo3d_amd.capi import STAGES
+
+    lib = co.load()
+    if lib.opo_device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is only used for the device-wide barrier
+        sync = lambda: None  # noqa: E731
+
+    grid, p, ph, fields = configs.cavity_64(n_round_trips=args.warmup + args.steps)
+    cells = grid.nx * grid.ny * grid.nt
+    S = p.n_slices
+    s = co.Solver(grid, p, device=0)
+    s.set_fields(*fields)
+    s.runOPO(args.warmup)
+    s.sync()
+    sync()
+    t0 = time.perf_counter()
+    pw = s.runOPO(args.steps)
+    s.sync()
+    sync()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    value = cells * S * args.steps / dt
+
+    # per-stage kernel time with HIP events on the solve's stream
+    ms = (C.c_double * 6)()
+    nl = (C.c_int * 6)()
+    co.check(lib.opo_profile_trips(s.h, args.profile_trips, ms, nl), "opo_profile_trips")
+    stages = {}
+    for k, name in enumerate(STAGES):
+        if nl[k] == 0:
+            continue
+        avg_ms = ms[k] / nl[k]
+        alg = cells * STAGE_FIELDS[name] * 8 * (STAGE_RW.get(name, 2))
+        stages[name] = {"launches": nl[k], "avg_us": avg_ms * 1e3, "total_ms": ms[k],
+                        "alg_bytes": alg, "GBs": alg / (avg_ms * 1e-3) / 1e9}
+    dom = max(stages, key=lambda k: stages[k]["total_ms"])
+    d = stages[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBs"], 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": None}
+    gpu_kernel_ms = sum(v["total_ms"] for v in stages.values()) / args.profile_trips
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64 (f32)",
+        "data": "synthetic (focused Gaussian pump + seeded Philox noise)",
+        "config": {"workload": "singly-resonant cavity 64x64x256, MgO:PPLN 5 mm, 100 slices/trip, "
+                               "Rs=0.98, GDD, free space; step = one round trip",
+                   "grid": [grid.nx, grid.ny, grid.nt], "slices": S},
+        "roofline": roofline,
+        "stages": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                   for k, v in stages.items()},
+        "gpu_kernel_ms_per_trip": gpu_kernel_ms,
+        "signal_power_last_trip_W": float(pw[-1, 1]),
+    }
+
+    if not args.no_cpu:
+        from oracle.pyoracle import Oracle, get_threads
+        g2, p2, _, f2 = configs.cavity_64(n_round_trips=args.cpu_trips)
+        o = Oracle(g2, p2)
+        o.set_fields(*f2)
+        c0 = time.perf_counter()
+        pr = o.run_round_trips(args.cpu_trips)
+        c1 = time.perf_counter()
+        cpu_rate = cells * S * args.cpu_trips / (c1 - c0)
+        # parity of the same trips on the GPU
+        s2 = co.Solver(g2, p2, device=0)
+        s2.set_fields(*f2)
+        pg = s2.runOPO(args.cpu_trips)
+        gf = s2.get_fields()
+        rf = o.get_fields()
+        errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(gf, rf)]
+        perr = float(np.max(np.abs(pg[:, 1] - pr[:, 1]) / np.abs(pr[:, 1])))
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-steps/s", "cores": get_threads(),
+                               "kind": "port",
+                               "sample": f"FP64 oracle, first {args.cpu_trips} round trips of the "
+                                         f"same 64x64x256 cavity workload ({c1 - c0:.1f} s)"}
+        out["parity_vs_oracle"] = {"trips": args.cpu_trips, "field_rel_l2_p_s_i": errs,
+                                   "signal_power_rel_max": perr}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

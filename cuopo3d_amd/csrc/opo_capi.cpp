@@ -1,0 +1,878 @@
+// opo_capi.cpp -- C ABI of the split-step path (include/opo3d.h).
+//
+// One opo_solve owns the device state of one OPO solve: the injected pump
+// Api, the fields Ap, As, Ai (layout [t][y][x], interleaved complex), the
+// propagator tables (built once in FP64, phases reduced mod 2*pi, then
+// rounded to the solve's precision) and the power/NaN bookkeeping.  Work is
+// queued on the solve's own stream; a round trip is captured once as a HIP
+// graph and replayed.
+//
+// Slice schedule (reference SSFM, Solver3D.cuh:946-960):
+//   N(h) D Q N(h) | N(h) D Q N(h) | ... is run as
+//   [N]  [D] [Q]  [N N] [D] [Q] ... [N N] [D] [Q]  [N]
+// where consecutive half steps of adjacent slices share one pointwise pass.
+// Within a round trip the first pass reads the pump from Api (re-injection,
+// Solver3D.cuh:981) and the last one applies the mirror factors
+// (Cavity3D.cuh:12-28), which commute with the free-space and GDD operators
+// that follow.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/opo3d.h"
+#include "../../include/opo3d_setup.h"
+#include "opo_device.hpp"
+#include "opo_internal.hpp"
+#include "opo_kernels.hpp"
+
+namespace opo {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+void clear_error() { g_err.clear(); }
+
+static const double kTwoPi = 6.283185307179586476925286766559;
+
+void cis(double phi, double &c, double &s) {
+    double r = std::fmod(phi, kTwoPi);
+    c = std::cos(r);
+    s = std::sin(r);
+}
+
+void omega_rule(int nt, double W, double *w) {
+    for (int i = 0; i < nt; ++i) w[i] = 0.0;
+    if (nt < 2) return;  // the reference leaves w[0] = 0 for NT = 1
+    std::vector<double> F(nt);
+    for (int i = 0; i < nt; ++i) F[i] = -0.5 * nt / W + i * ((double)nt / W) / (nt - 1);
+    const int c = nt / 2;
+    for (int i = 0; i < c; ++i) {
+        w[i + c] = F[i];
+        w[i] = F[i + c];
+    }
+    for (int i = 0; i < nt; ++i) w[i] *= kTwoPi;
+}
+
+#define CK(call)                                                                           \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(OPO_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_),       \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+// exp(-2 pi i j/n) from long double on exact fractions.
+static void unit_root_neg(long j, long n, double &c, double &s) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    long double a = -2.0L * pi * (long double)(j % n) / (long double)n;
+    c = (double)cosl(a);
+    s = (double)sinl(a);
+}
+
+// Signed frequency index of FFT-order bin j: the reference builds centred
+// tables and fftShift2D's them (Solver3D.cuh:288-296, EFields3D.cuh:141-171).
+static double signed_freq(int j, int n) {
+    if (n < 2) return 0.0;
+    int c = n / 2;
+    return (double)(((j + c) % n) - c);
+}
+
+static bool grid_supported(const opo_grid &g, bool fp64) {
+    if (!tline_supported(g.nt, fp64)) return false;
+    if (!xyplane_supported(g.nx, g.ny, fp64)) return false;
+    return true;
+}
+
+// -------------------------------------------------------------- solve
+struct SolveImpl {
+    opo_grid g{};
+    opo_params p{};
+    std::vector<double> omega;
+    int device = 0;
+    int prec = OPO_FP32;
+    size_t N = 0, P = 0;
+    int trips_done = 0;
+    hipStream_t stream = nullptr;
+    bool use_graphs = true;
+    virtual ~SolveImpl() {}
+    virtual int init() = 0;
+    virtual int set_fields(const void *, const void *, const void *) = 0;
+    virtual int get_fields(void *, void *, void *) = 0;
+    virtual void *dev_ptr(int which) = 0;
+    virtual int step(int which) = 0;
+    virtual int ssfm() = 0;
+    virtual int enqueue_trips(int n) = 0;
+    virtual int finish_trips(int n, double *power) = 0;
+    virtual int power(double out[3]) = 0;
+    virtual int profile_trips(int n, double *ms, int *launches) = 0;
+    size_t field_bytes() const { return N * (prec == OPO_FP64 ? 16 : 8); }
+    int select() {
+        CK(hipSetDevice(device));
+        return OPO_OK;
+    }
+};
+
+enum { STEP_NONLINEAR, STEP_DISPERSION, STEP_DIFFRACTION, STEP_FREE_SPACE, STEP_GDD, STEP_MIRROR };
+
+template <typename T> struct SolveT final : SolveImpl {
+    using C = cplx<T>;
+    C *Api = nullptr, *A[3] = {nullptr, nullptr, nullptr};
+    C *tables = nullptr;
+    C *tw_x = nullptr, *tw_t = nullptr, *tabT[3] = {}, *tabG = nullptr, *qx[3] = {}, *qy[3] = {},
+      *qfsx[2] = {}, *qfsy[2] = {};
+    double *dbuf = nullptr;   // partials | pw3
+    double *partial = nullptr, *pw3 = nullptr;
+    double *series = nullptr;
+    int series_cap = 0;
+    int *flags = nullptr;     // [0] status, [1] trip counter
+    int nblk = 64;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+
+    ~SolveT() override {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &gx : graph)
+            if (gx) (void)hipGraphExecDestroy(gx);
+        (void)hipFree(Api);
+        for (auto *a : A) (void)hipFree(a);
+        (void)hipFree(tables);
+        (void)hipFree(dbuf);
+        (void)hipFree(series);
+        (void)hipFree(flags);
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int init() override {
+        P = (size_t)g.nx * g.ny;
+        N = P * g.nt;
+        int rc = select();
+        if (rc) return rc;
+        CK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        const size_t fb = N * sizeof(C);
+        if (hipMalloc(&Api, fb) != hipSuccess) return fail(OPO_ERR_NOMEM, "hipMalloc(%zu) failed", fb);
+        for (auto &a : A)
+            if (hipMalloc(&a, fb) != hipSuccess) return fail(OPO_ERR_NOMEM, "hipMalloc(%zu) failed", fb);
+        CK(hipMemsetAsync(Api, 0, fb, stream));
+        for (auto *a : A) CK(hipMemsetAsync(a, 0, fb, stream));
+        // power partials: enough blocks to spread the sum, fixed per grid.
+        size_t want = (N + 256 * 16 - 1) / (256 * 16);
+        nblk = (int)(want < 1 ? 1 : (want > 1024 ? 1024 : want));
+        CK(hipMalloc(&dbuf, sizeof(double) * (3 * (size_t)nblk + 4)));
+        partial = dbuf;
+        pw3 = dbuf + 3 * (size_t)nblk;
+        CK(hipMalloc(&flags, 4 * sizeof(int)));
+        CK(hipMemsetAsync(flags, 0, 4 * sizeof(int), stream));
+        rc = build_tables();
+        if (rc) return rc;
+        CK(hipStreamSynchronize(stream));
+        const char *ng = std::getenv("OPO_NO_GRAPH");
+        use_graphs = !(ng && ng[0] == '1');
+        return OPO_OK;
+    }
+
+    int build_tables() {
+        const int nx = g.nx, ny = g.ny, nt = g.nt;
+        // Every entry is stored as a (hi, lo) pair: hi = value rounded to T,
+        // lo = rounding residual (zero for T = double).
+        const size_t total = 2 * ((size_t)nx + nt + 3 * nt + nt + 3 * nx + 3 * ny + 2 * nx + 2 * ny);
+        std::vector<C> h(total);
+        size_t off = 0;
+        auto take = [&](size_t n) { size_t o = off; off += 2 * n; return o; };
+        const size_t o_twx = take(nx), o_twt = take(nt), o_T = take(3 * (size_t)nt), o_G = take(nt),
+                     o_qx = take(3 * (size_t)nx), o_qy = take(3 * (size_t)ny),
+                     o_fx = take(2 * (size_t)nx), o_fy = take(2 * (size_t)ny);
+        // put(i, re, im): the pair of one entry at h[i] (hi) and h[i + 1] (lo);
+        // entry j of a table starting at o lives at i = o + 2 j.
+        auto put = [&](size_t i, double re, double im) {
+            const T hr = (T)re, hi = (T)im;
+            h[i] = C{hr, hi};
+            h[i + 1] = C{(T)(re - (double)hr), (T)(im - (double)hi)};
+        };
+        for (int j = 0; j < nx; ++j) {
+            double c, s;
+            unit_root_neg(j, nx, c, s);
+            put(o_twx + 2 * (size_t)j, c, s);
+        }
+        for (int j = 0; j < nt; ++j) {
+            double c, s;
+            unit_root_neg(j, nt, c, s);
+            put(o_twt + 2 * (size_t)j, c, s);
+        }
+        const double *w = omega.data();
+        // D: eiLz_X / NT (Solver3D.cuh:153-169; the 1/NT of the time pair folded in).
+        for (int f = 0; f < 3; ++f) {
+            const double att = std::exp(-0.5 * p.alpha[f] * p.dz) / nt;
+            for (int k = 0; k < nt; ++k) {
+                double c, s;
+                cis(p.dz * w[k] * (p.delta[f] + 0.5 * w[k] * p.beta2[f]), c, s);
+                put(o_T + 2 * ((size_t)f * nt + k), c * att, s * att);
+            }
+        }
+        // GDD: exp(i w^2 GDD / 2) / NT (Solver3D.cuh:261-277, 676).
+        for (int k = 0; k < nt; ++k) {
+            double c, s;
+            cis(0.5 * w[k] * w[k] * p.gdd, c, s);
+            put(o_G + 2 * (size_t)k, c / nt, s / nt);
+        }
+        // Q: exp(-i dz 2 pi^2/k (dfX^2 fx^2 + dfY^2 fy^2)) exp(-alpha dz/2) /(NX NY)
+        // (Solver3D.cuh:282-301), split exactly into an x factor (carrying the
+        // absorption and 1/NX) and a y factor (carrying 1/NY).
+        const double dfX = nx > 1 ? 1.0 / (p.dx * nx) : 0.0;
+        const double dfY = ny > 1 ? 1.0 / (p.dy * ny) : 0.0;
+        const double pi2 = 9.869604401089358618834490999876151;  // pi^2
+        for (int f = 0; f < 3; ++f) {
+            const double att = std::exp(-0.5 * p.alpha[f] * p.dz);
+            for (int j = 0; j < nx; ++j) {
+                double fx = signed_freq(j, nx), c, s;
+                cis(-p.dz * (2.0 * pi2 / p.kdiff[f] * (dfX * dfX * fx * fx)), c, s);
+                put(o_qx + 2 * ((size_t)f * nx + j), c * att / nx, s * att / nx);
+            }
+            for (int j = 0; j < ny; ++j) {
+                double fy = signed_freq(j, ny), c, s;
+                cis(-p.dz * (2.0 * pi2 / p.kdiff[f] * (dfY * dfY * fy * fy)), c, s);
+                put(o_qy + 2 * ((size_t)f * ny + j), c / ny, s / ny);
+            }
+        }
+        // Free space on s and i: exp(+i L 2 pi^2/k (dfX^2 fx^2 + dfY^2 fy^2)), no
+        // absorption (Solver3D.cuh:305-323), normalised by 1/(NX NY) (:102-119).
+        for (int f = 0; f < 2; ++f) {
+            for (int j = 0; j < nx; ++j) {
+                double fx = signed_freq(j, nx), c, s;
+                cis(p.fs_length * (2.0 * pi2 / p.kdiff[1 + f] * (dfX * dfX * fx * fx)), c, s);
+                put(o_fx + 2 * ((size_t)f * nx + j), c / nx, s / nx);
+            }
+            for (int j = 0; j < ny; ++j) {
+                double fy = signed_freq(j, ny), c, s;
+                cis(p.fs_length * (2.0 * pi2 / p.kdiff[1 + f] * (dfY * dfY * fy * fy)), c, s);
+                put(o_fy + 2 * ((size_t)f * ny + j), c / ny, s / ny);
+            }
+        }
+        if (hipMalloc(&tables, total * sizeof(C)) != hipSuccess)
+            return fail(OPO_ERR_NOMEM, "hipMalloc(tables) failed");
+        CK(hipMemcpy(tables, h.data(), total * sizeof(C), hipMemcpyHostToDevice));
+        tw_x = tables + o_twx;
+        tw_t = tables + o_twt;
+        for (int f = 0; f < 3; ++f) {
+            tabT[f] = tables + o_T + 2 * (size_t)f * nt;
+            qx[f] = tables + o_qx + 2 * (size_t)f * nx;
+            qy[f] = tables + o_qy + 2 * (size_t)f * ny;
+        }
+        tabG = tables + o_G;
+        for (int f = 0; f < 2; ++f) {
+            qfsx[f] = tables + o_fx + 2 * (size_t)f * nx;
+            qfsy[f] = tables + o_fy + 2 * (size_t)f * ny;
+        }
+        return OPO_OK;
+    }
+
+    // RK4 arguments for nsteps consecutive half steps at slice positions z0, z1.
+    Rk4Args<T> rk_args(int nsteps, double z0, double z1, bool mirror) const {
+        Rk4Args<T> a{};
+        const double h = 0.5 * p.dz;
+        a.kp = (T)p.kappa[0];
+        a.ks = (T)p.kappa[1];
+        a.ki = (T)p.kappa[2];
+        a.c1 = (T)(p.rk4_h_in_stages ? 0.5 * h : 0.5);
+        a.c3 = (T)(p.rk4_h_in_stages ? h : 1.0);
+        a.h = (T)h;
+        a.nsteps = nsteps;
+        a.has_phase = (p.dk_phase_scale != 0.0 && p.dk != 0.0) ? 1 : 0;
+        for (int st = 0; st < 2; ++st) {
+            const double z = st == 0 ? z0 : z1;
+            // stage positions z, z + dz/4, z + dz/2 (Solver3D.cuh:911-929)
+            const double zs[3] = {z, z + 0.25 * p.dz, z + 0.5 * p.dz};
+            for (int k = 0; k < 3; ++k) {
+                double c = 1.0, s = 0.0;
+                if (a.has_phase) cis(p.dk_phase_scale * p.dk * zs[k], c, s);
+                a.es_re[st][k] = (T)c;
+                a.es_im[st][k] = (T)s;
+            }
+        }
+        a.out_scale = mirror ? 1 : 0;
+        a.ms_re = (T)p.fb_re[0];
+        a.ms_im = (T)p.fb_im[0];
+        a.mi_re = (T)p.fb_re[1];
+        a.mi_im = (T)p.fb_im[1];
+        a.ms_lo_re = (T)(p.fb_re[0] - (double)a.ms_re);
+        a.ms_lo_im = (T)(p.fb_im[0] - (double)a.ms_im);
+        a.mi_lo_re = (T)(p.fb_re[1] - (double)a.mi_re);
+        a.mi_lo_im = (T)(p.fb_im[1] - (double)a.mi_im);
+        return a;
+    }
+
+    Geometry geo() const { return Geometry{g.nx, g.ny, g.nt}; }
+
+    // Stage timing (opo_profile_trips): when profiling, every launch is
+    // bracketed by a pair of events recorded on the solve's stream.
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_stage;  // stage of the pair (2k, 2k+1)
+    size_t ev_used = 0;
+    int prof_mark(int stage) {
+        if (!profiling) return OPO_OK;
+        if (ev_used + 1 >= ev_pool.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                CK(hipEventCreate(&e));
+                ev_pool.push_back(e);
+            }
+        }
+        if (ev_used % 2 == 0) ev_stage.push_back(stage);
+        CK(hipEventRecord(ev_pool[ev_used++], stream));
+        return OPO_OK;
+    }
+#define PROF(stage, launch_expr)                 \
+    do {                                         \
+        int prc_ = prof_mark(stage);             \
+        if (prc_) return prc_;                   \
+        CK(launch_expr);                         \
+        if ((prc_ = prof_mark(stage))) return prc_; \
+    } while (0)
+
+    int q_rk4(const C *psrc, const Rk4Args<T> &a, const int *st) {
+        PROF(OPO_STAGE_NONLINEAR, (launch_rk4<T>(stream, psrc, A[0], A[1], A[2], N, a, st)));
+        return OPO_OK;
+    }
+    int q_dispersion(const int *st) {
+        FieldOps<T> ops{};
+        for (int f = 0; f < 3; ++f) { ops.f[f] = A[f]; ops.tab[f] = tabT[f]; }
+        PROF(OPO_STAGE_TIME_OP, (launch_tline<T>(stream, geo(), ops, 3, tw_t, st)));
+        return OPO_OK;
+    }
+    int q_gdd(const int *st) {
+        FieldOps<T> ops{};
+        for (int f = 0; f < 2; ++f) { ops.f[f] = A[1 + f]; ops.tab[f] = tabG; }
+        PROF(OPO_STAGE_GDD, (launch_tline<T>(stream, geo(), ops, 2, tw_t, st)));
+        return OPO_OK;
+    }
+    int q_diffraction(const int *st) {
+        FieldOps<T> ops{};
+        for (int f = 0; f < 3; ++f) { ops.f[f] = A[f]; ops.tab[f] = qx[f]; ops.taby[f] = qy[f]; }
+        PROF(OPO_STAGE_XY_OP, (launch_xyplane<T>(stream, geo(), ops, 3, tw_x, st)));
+        return OPO_OK;
+    }
+    int q_free_space(const int *st) {
+        FieldOps<T> ops{};
+        for (int f = 0; f < 2; ++f) { ops.f[f] = A[1 + f]; ops.tab[f] = qfsx[f]; ops.taby[f] = qfsy[f]; }
+        PROF(OPO_STAGE_FREE_SPACE, (launch_xyplane<T>(stream, geo(), ops, 2, tw_x, st)));
+        return OPO_OK;
+    }
+
+    // SSFM (Solver3D.cuh:946-960) on the fused schedule.  trip: read the pump
+    // from Api and apply the mirror factors at the end.
+    int q_ssfm(bool trip, const int *st) {
+        const int S = p.n_slices;
+        int rc;
+        if (S <= 0) {
+            if (trip) CK(hipMemcpyAsync(A[0], Api, N * sizeof(C), hipMemcpyDeviceToDevice, stream));
+            return OPO_OK;
+        }
+        for (int k = 0; k < S; ++k) {
+            const double z = k * p.dz;
+            if (k == 0) rc = q_rk4(trip ? Api : A[0], rk_args(1, z, 0.0, false), st);
+            else rc = q_rk4(A[0], rk_args(2, (k - 1) * p.dz, z, false), st);
+            if (rc) return rc;
+            if ((rc = q_dispersion(st))) return rc;
+            if ((rc = q_diffraction(st))) return rc;
+        }
+        return q_rk4(A[0], rk_args(1, (S - 1) * p.dz, 0.0, trip), st);
+    }
+
+    // One round trip (Solver3D.cuh:979-1011) with the per-trip power.
+    int q_trip(bool fs) {
+        int rc = q_ssfm(true, flags);
+        if (rc) return rc;
+        if (fs && (rc = q_free_space(flags))) return rc;
+        if (p.gdd_on && (rc = q_gdd(flags))) return rc;
+        int prc = prof_mark(OPO_STAGE_POWER);
+        if (prc) return prc;
+        CK(launch_power_partial<T>(stream, A[0], A[1], A[2], N, nblk, partial, flags));
+        CK(launch_power_final(stream, partial, nblk, p.power_prefactor[0], p.power_prefactor[1],
+                              p.power_prefactor[2], series, flags + 1, flags, 1, series_cap));
+        return prof_mark(OPO_STAGE_POWER);
+    }
+
+    // n trips without graphs, events around every launch; sums per stage.
+    int profile_trips(int n, double *ms, int *launches) override {
+        int rc = select();
+        if (rc) return rc;
+        for (int k = 0; k < OPO_N_STAGES; ++k) { ms[k] = 0.0; launches[k] = 0; }
+        if ((rc = ensure_series(n))) return rc;
+        CK(hipMemsetAsync(flags, 0, 2 * sizeof(int), stream));
+        profiling = true;
+        for (int i = 0; i < n && !rc; ++i) {
+            ev_used = 0;
+            ev_stage.clear();
+            const int rt = trips_done;
+            const bool fs = p.fs_on && !(p.skip_fs_on_last_trip && rt == p.n_round_trips - 1);
+            rc = q_trip(fs);
+            if (rc) break;
+            hipError_t e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) { rc = fail(OPO_ERR_HIP, "profile sync: %s", hipGetErrorString(e)); break; }
+            for (size_t k = 0; k + 1 < ev_used; k += 2) {
+                float t = 0.f;
+                e = hipEventElapsedTime(&t, ev_pool[k], ev_pool[k + 1]);
+                if (e != hipSuccess) { rc = fail(OPO_ERR_HIP, "event time: %s", hipGetErrorString(e)); break; }
+                const int st = ev_stage[k / 2];
+                ms[st] += t;
+                launches[st] += 1;
+            }
+            trips_done++;
+        }
+        profiling = false;
+        return rc;
+    }
+
+    int set_fields(const void *api, const void *as, const void *ai) override {
+        int rc = select();
+        if (rc) return rc;
+        const size_t fb = N * sizeof(C);
+        CK(hipStreamSynchronize(stream));
+        if (api) {
+            CK(hipMemcpy(Api, api, fb, hipMemcpyHostToDevice));
+            CK(hipMemcpy(A[0], Api, fb, hipMemcpyDeviceToDevice));
+        }
+        if (as) CK(hipMemcpy(A[1], as, fb, hipMemcpyHostToDevice));
+        if (ai) CK(hipMemcpy(A[2], ai, fb, hipMemcpyHostToDevice));
+        return OPO_OK;
+    }
+    int get_fields(void *ap, void *as, void *ai) override {
+        int rc = select();
+        if (rc) return rc;
+        const size_t fb = N * sizeof(C);
+        CK(hipStreamSynchronize(stream));
+        void *dst[3] = {ap, as, ai};
+        for (int f = 0; f < 3; ++f)
+            if (dst[f]) CK(hipMemcpy(dst[f], A[f], fb, hipMemcpyDeviceToHost));
+        return OPO_OK;
+    }
+    void *dev_ptr(int which) override {
+        if (which >= 0 && which < 3) return A[which];
+        if (which == 3) return Api;
+        return nullptr;
+    }
+
+    int step(int which) override {
+        int rc = select();
+        if (rc) return rc;
+        switch (which) {
+            case STEP_NONLINEAR: rc = q_rk4(A[0], rk_args(1, 0.0, 0.0, false), nullptr); break;
+            case STEP_DISPERSION: rc = q_dispersion(nullptr); break;
+            case STEP_DIFFRACTION: rc = q_diffraction(nullptr); break;
+            case STEP_FREE_SPACE: rc = q_free_space(nullptr); break;
+            case STEP_GDD: rc = p.gdd_on ? q_gdd(nullptr) : OPO_OK; break;
+            case STEP_MIRROR: {
+                // mirror alone: a zero-coupling RK4 pass is the identity, then the factors.
+                Rk4Args<T> a = rk_args(1, 0.0, 0.0, true);
+                a.kp = a.ks = a.ki = (T)0;
+                rc = q_rk4(A[0], a, nullptr);
+                break;
+            }
+            default: return fail(OPO_ERR_ARG, "unknown step %d", which);
+        }
+        if (rc) return rc;
+        CK(hipStreamSynchronize(stream));
+        return OPO_OK;
+    }
+
+    int ssfm() override {
+        int rc = select();
+        if (rc) return rc;
+        if ((rc = q_ssfm(false, nullptr))) return rc;
+        CK(hipStreamSynchronize(stream));
+        return OPO_OK;
+    }
+
+    int ensure_series(int n) {
+        if (n <= series_cap) return OPO_OK;
+        for (auto &gx : graph)
+            if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+        CK(hipStreamSynchronize(stream));
+        (void)hipFree(series);
+        series = nullptr;
+        int cap = n < 64 ? 64 : n;
+        if (hipMalloc(&series, sizeof(double) * 3 * (size_t)cap) != hipSuccess)
+            return fail(OPO_ERR_NOMEM, "hipMalloc(series) failed");
+        series_cap = cap;
+        return OPO_OK;
+    }
+
+    int launch_trip(bool fs) {
+        int v = fs ? 0 : 1;
+        if (!use_graphs) return q_trip(fs);
+        if (!graph[v]) {
+            hipGraph_t gr = nullptr;
+            CK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            int rc = q_trip(fs);
+            hipError_t e = hipStreamEndCapture(stream, &gr);
+            if (rc) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(OPO_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&graph[v], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) return fail(OPO_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        }
+        CK(hipGraphLaunch(graph[v], stream));
+        return OPO_OK;
+    }
+
+    int pending = 0;
+    int enqueue_trips(int n) override {
+        int rc = select();
+        if (rc) return rc;
+        if ((rc = ensure_series(n))) return rc;
+        CK(hipMemsetAsync(flags, 0, 2 * sizeof(int), stream));
+        for (int i = 0; i < n; ++i) {
+            const int rt = trips_done + i;
+            const bool fs = p.fs_on && !(p.skip_fs_on_last_trip && rt == p.n_round_trips - 1);
+            if ((rc = launch_trip(fs))) return rc;
+        }
+        pending = n;
+        return OPO_OK;
+    }
+    int finish_trips(int n, double *power) override {
+        int rc = select();
+        if (rc) return rc;
+        int hflags[2] = {0, 0};
+        CK(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, stream));
+        std::vector<double> hs(3 * (size_t)n);
+        if (power && n > 0)
+            CK(hipMemcpyAsync(hs.data(), series, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost,
+                              stream));
+        CK(hipStreamSynchronize(stream));
+        pending = 0;
+        const int done = hflags[0] ? hflags[0] : n;
+        if (power) std::memcpy(power, hs.data(), sizeof(double) * 3 * (size_t)done);
+        trips_done += done;
+        if (hflags[0])
+            return fail(OPO_ERR_NAN, "non-finite signal power at round trip %d", trips_done - 1);
+        return OPO_OK;
+    }
+
+    int power(double out[3]) override {
+        int rc = select();
+        if (rc) return rc;
+        CK(launch_power_partial<T>(stream, A[0], A[1], A[2], N, nblk, partial, nullptr));
+        CK(launch_power_final(stream, partial, nblk, p.power_prefactor[0], p.power_prefactor[1],
+                              p.power_prefactor[2], pw3, nullptr, nullptr, 0, 1));
+        CK(hipMemcpyAsync(out, pw3, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        CK(hipStreamSynchronize(stream));
+        return OPO_OK;
+    }
+};
+
+static int validate(const opo_grid *g, const opo_params *p) {
+    if (!g || !p) return fail(OPO_ERR_ARG, "null grid or params");
+    if (g->nx < 1 || g->ny < 1 || g->nt < 1) return fail(OPO_ERR_ARG, "grid dimensions must be >= 1");
+    if (p->precision != OPO_FP32 && p->precision != OPO_FP64)
+        return fail(OPO_ERR_ARG, "precision must be OPO_FP32 or OPO_FP64");
+    if (!grid_supported(*g, p->precision == OPO_FP64))
+        return fail(OPO_ERR_UNSUPPORTED, "grid %dx%dx%d (%s) is not supported", g->nx, g->ny, g->nt,
+                    p->precision == OPO_FP64 ? "fp64" : "fp32");
+    if (!(p->dz > 0) || p->n_slices < 0) return fail(OPO_ERR_ARG, "dz must be > 0 and n_slices >= 0");
+    if (!p->omega && g->nt > 1 && !(p->t_window > 0))
+        return fail(OPO_ERR_ARG, "t_window must be > 0 when omega is not given");
+    for (int f = 0; f < 3; ++f)
+        if (!(p->kdiff[f] > 0)) return fail(OPO_ERR_ARG, "kdiff must be > 0");
+    if ((g->nx > 1 && !(p->dx > 0)) || (g->ny > 1 && !(p->dy > 0)))
+        return fail(OPO_ERR_ARG, "dx, dy must be > 0");
+    return OPO_OK;
+}
+
+static int make_solve(const opo_grid *g, const opo_params *p, int device,
+                      std::unique_ptr<SolveImpl> &out) {
+    int rc = validate(g, p);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(OPO_ERR_NOGPU, "no HIP device");
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) device = 0;
+    }
+    if (device >= ndev) return fail(OPO_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+    std::unique_ptr<SolveImpl> s;
+    if (p->precision == OPO_FP64) s.reset(new SolveT<double>());
+    else s.reset(new SolveT<float>());
+    s->g = *g;
+    s->p = *p;
+    s->p.omega = nullptr;
+    s->prec = p->precision;
+    s->device = device;
+    s->omega.assign(g->nt, 0.0);
+    if (p->omega) for (int i = 0; i < g->nt; ++i) s->omega[i] = p->omega[i];
+    else omega_rule(g->nt, p->t_window, s->omega.data());
+    rc = s->init();
+    if (rc) return rc;
+    out = std::move(s);
+    return OPO_OK;
+}
+
+}  // namespace opo
+
+using namespace opo;
+
+struct opo_solve {
+    std::unique_ptr<SolveImpl> s;
+};
+
+// ----------------------------------------------------------------- batch
+struct BatchEntry {
+    uint64_t ticket = 0;
+    opo_grid g{};
+    opo_params p{};
+    std::vector<unsigned char> api, as, ai;  // host copies while queued
+    std::unique_ptr<SolveImpl> s;
+    int n_trips = 0, done_trips = 0;
+    std::vector<double> power;
+    size_t footprint = 0;
+    bool finished = false;
+    int error = OPO_OK;
+    std::string error_msg;
+};
+
+struct opo_batch {
+    int device = 0;
+    size_t budget = 0, used = 0;
+    int prec = OPO_FP32;
+    uint64_t next_ticket = 1;
+    std::deque<BatchEntry *> queue;
+    std::vector<BatchEntry *> resident;
+    std::map<uint64_t, std::unique_ptr<BatchEntry>> all;
+    std::mutex mu;
+};
+
+extern "C" {
+
+const char *opo_last_error(void) { return g_err.c_str(); }
+const char *opo_version(void) { return "opo3d-mi355x 0.1 (gfx950)"; }
+int opo_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int opo_create(const opo_grid *grid, const opo_params *params, int device, opo_solve **out) {
+    if (!out) return fail(OPO_ERR_ARG, "opo_create: null output handle");
+    *out = nullptr;
+    std::unique_ptr<SolveImpl> s;
+    int rc = make_solve(grid, params, device, s);
+    if (rc) return rc;
+    *out = new opo_solve{std::move(s)};
+    return OPO_OK;
+}
+
+void opo_destroy(opo_solve *s) { delete s; }
+
+#define SOLVE_OR_FAIL(s) \
+    if (!(s) || !(s)->s) return fail(OPO_ERR_ARG, "null solve handle")
+
+int opo_set_fields(opo_solve *s, const void *api, const void *as, const void *ai) {
+    SOLVE_OR_FAIL(s);
+    return s->s->set_fields(api, as, ai);
+}
+int opo_get_fields(opo_solve *s, void *ap, void *as, void *ai) {
+    SOLVE_OR_FAIL(s);
+    return s->s->get_fields(ap, as, ai);
+}
+void *opo_field_device_ptr(opo_solve *s, int which) {
+    if (!s || !s->s) return nullptr;
+    return s->s->dev_ptr(which);
+}
+size_t opo_field_bytes(const opo_solve *s) { return (s && s->s) ? s->s->field_bytes() : 0; }
+
+int opo_step_nonlinear(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->step(STEP_NONLINEAR); }
+int opo_step_dispersion(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->step(STEP_DISPERSION); }
+int opo_step_diffraction(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->step(STEP_DIFFRACTION); }
+int opo_step_free_space(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->step(STEP_FREE_SPACE); }
+int opo_step_gdd(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->step(STEP_GDD); }
+int opo_step_mirror(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->step(STEP_MIRROR); }
+int opo_ssfm(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->ssfm(); }
+int opo_single_pass(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->ssfm(); }
+
+int opo_run_round_trips(opo_solve *s, int n, double *power) {
+    SOLVE_OR_FAIL(s);
+    if (n < 0) return fail(OPO_ERR_ARG, "n must be >= 0");
+    if (n == 0) return OPO_OK;
+    int rc = s->s->enqueue_trips(n);
+    if (rc) return rc;
+    return s->s->finish_trips(n, power);
+}
+int opo_trips_done(const opo_solve *s) { return (s && s->s) ? s->s->trips_done : 0; }
+int opo_power(opo_solve *s, double out[3]) {
+    SOLVE_OR_FAIL(s);
+    if (!out) return fail(OPO_ERR_ARG, "null output");
+    return s->s->power(out);
+}
+int opo_courant_ok(const opo_params *p) {
+    if (!p) return 0;
+    // Solver3D.cuh:495-504, 973: dz < 0.2 kp dx dy
+    return p->dz < 0.2 * p->kdiff[0] * p->dx * p->dy ? 1 : 0;
+}
+int opo_profile_trips(opo_solve *s, int n, double ms[OPO_N_STAGES], int launches[OPO_N_STAGES]) {
+    SOLVE_OR_FAIL(s);
+    if (n < 1 || !ms || !launches) return fail(OPO_ERR_ARG, "opo_profile_trips: bad argument");
+    return s->s->profile_trips(n, ms, launches);
+}
+int opo_sync(opo_solve *s) {
+    SOLVE_OR_FAIL(s);
+    int rc = s->s->select();
+    if (rc) return rc;
+    CK(hipStreamSynchronize(s->s->stream));
+    return OPO_OK;
+}
+
+// ------------------------------------------------------------- batching
+size_t opo_batch_footprint(const opo_grid *g, int precision) {
+    if (!g) return 0;
+    const size_t n = (size_t)g->nx * g->ny * g->nt;
+    const size_t cb = precision == OPO_FP64 ? 16 : 8;
+    const size_t tabs = (size_t)(3 * g->nx + 3 * g->ny + 5 * g->nx + 5 * g->nt + 2 * g->ny + 64) * cb;
+    return 4 * n * cb + tabs + 64 * 1024;
+}
+
+int opo_batch_create(int device, size_t budget, int precision, opo_batch **out) {
+    if (!out) return fail(OPO_ERR_ARG, "null output");
+    *out = nullptr;
+    int ndev = opo_device_count();
+    if (ndev < 1) return fail(OPO_ERR_NOGPU, "no HIP device");
+    if (device < 0) device = 0;
+    if (device >= ndev) return fail(OPO_ERR_ARG, "device out of range");
+    auto *b = new opo_batch;
+    b->device = device;
+    b->budget = budget;
+    b->prec = precision;
+    *out = b;
+    return OPO_OK;
+}
+
+void opo_batch_destroy(opo_batch *b) { delete b; }
+
+int opo_batch_submit(opo_batch *b, const opo_grid *g, const opo_params *p, const void *api,
+                     const void *as, const void *ai, int n_trips, uint64_t *ticket) {
+    if (!b || !g || !p || !api || !as || !ai || n_trips < 1)
+        return fail(OPO_ERR_ARG, "opo_batch_submit: bad argument");
+    int rc = validate(g, p);
+    if (rc) return rc;
+    if (p->precision != b->prec) return fail(OPO_ERR_ARG, "precision differs from the batch's");
+    const size_t fp = opo_batch_footprint(g, p->precision);
+    if (fp > b->budget) return fail(OPO_ERR_BUDGET, "solve needs %zu bytes, budget is %zu", fp, b->budget);
+    std::lock_guard<std::mutex> lk(b->mu);
+    auto e = std::make_unique<BatchEntry>();
+    e->ticket = b->next_ticket++;
+    e->g = *g;
+    e->p = *p;
+    e->p.omega = nullptr;
+    e->n_trips = n_trips;
+    e->footprint = fp;
+    const size_t fb = (size_t)g->nx * g->ny * g->nt * (p->precision == OPO_FP64 ? 16 : 8);
+    e->api.assign((const unsigned char *)api, (const unsigned char *)api + fb);
+    e->as.assign((const unsigned char *)as, (const unsigned char *)as + fb);
+    e->ai.assign((const unsigned char *)ai, (const unsigned char *)ai + fb);
+    e->power.assign(3 * (size_t)n_trips, 0.0);
+    if (ticket) *ticket = e->ticket;
+    b->queue.push_back(e.get());
+    b->all[e->ticket] = std::move(e);
+    return OPO_OK;
+}
+
+int opo_batch_step(opo_batch *b, int *n_active) {
+    if (!b) return fail(OPO_ERR_ARG, "null batch");
+    std::lock_guard<std::mutex> lk(b->mu);
+    // admit in arrival order while the head fits
+    while (!b->queue.empty() && b->used + b->queue.front()->footprint <= b->budget) {
+        BatchEntry *e = b->queue.front();
+        b->queue.pop_front();
+        std::unique_ptr<SolveImpl> s;
+        int rc = make_solve(&e->g, &e->p, b->device, s);
+        if (!rc) rc = s->set_fields(e->api.data(), e->as.data(), e->ai.data());
+        if (rc) {
+            e->finished = true;
+            e->error = rc;
+            e->error_msg = g_err;
+            continue;
+        }
+        e->api.clear(); e->api.shrink_to_fit();
+        e->as.clear(); e->as.shrink_to_fit();
+        e->ai.clear(); e->ai.shrink_to_fit();
+        e->s = std::move(s);
+        b->used += e->footprint;
+        b->resident.push_back(e);
+    }
+    // one round trip of every resident solve, overlapped across streams
+    int stepped = 0;
+    for (BatchEntry *e : b->resident) {
+        int rc = e->s->enqueue_trips(1);
+        if (rc) { e->error = rc; e->error_msg = g_err; }
+    }
+    std::vector<BatchEntry *> keep;
+    for (BatchEntry *e : b->resident) {
+        if (!e->error) {
+            int rc = e->s->finish_trips(1, &e->power[3 * (size_t)e->done_trips]);
+            if (rc == OPO_OK) e->done_trips++;
+            else { e->error = rc; e->error_msg = g_err; }
+            ++stepped;
+        }
+        if (e->error || e->done_trips >= e->n_trips) {
+            e->finished = true;
+        } else {
+            keep.push_back(e);
+        }
+    }
+    // finished solves keep their device memory until polled
+    b->resident.swap(keep);
+    if (n_active) *n_active = stepped;
+    return OPO_OK;
+}
+
+int opo_batch_poll(opo_batch *b, uint64_t ticket, int *done, double *power, void *ap, void *as,
+                   void *ai) {
+    if (!b || !done) return fail(OPO_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    auto it = b->all.find(ticket);
+    if (it == b->all.end()) return fail(OPO_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    BatchEntry *e = it->second.get();
+    *done = e->finished ? 1 : 0;
+    if (!e->finished) return OPO_OK;
+    int rc = e->error;
+    std::string msg = e->error_msg;
+    if (power) std::memcpy(power, e->power.data(), sizeof(double) * 3 * (size_t)e->done_trips);
+    if (e->s) {
+        int r2 = e->s->get_fields(ap, as, ai);
+        if (!rc) rc = r2;
+        b->used -= e->footprint;
+        e->s.reset();
+    }
+    b->all.erase(it);
+    if (rc) return fail(rc, "%s", msg.c_str());
+    return OPO_OK;
+}
+
+int opo_batch_stats(opo_batch *b, int *resident, int *queued, size_t *bytes_used) {
+    if (!b) return fail(OPO_ERR_ARG, "null batch");
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (resident) *resident = (int)b->resident.size();
+    if (queued) *queued = (int)b->queue.size();
+    if (bytes_used) *bytes_used = b->used;
+    return OPO_OK;
+}
+
+}  // extern "C"

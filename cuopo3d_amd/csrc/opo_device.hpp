@@ -1,0 +1,338 @@
+// opo_device.hpp -- device building blocks of the split-step path on CDNA4.
+//
+//  * cplx<T>: interleaved (re, im) complex in T = float or double, the
+//    reference's complex_t layout (DtypesConsts.cuh:20).
+//  * Compile-time twiddles: cos/sin(2*pi*j/n) evaluated by the compiler
+//    from an exact octant reduction and a Taylor series, so every register
+//    DFT uses correctly rounded constants (no recurrences).
+//  * dft<N,S>: natural-order DFT of N values held in registers, exponent
+//    sign S, built from radix-2/3/4 codelets by the two-level split
+//    X[k1 + A k2] = sum_b w_B^{b k2} w_N^{b k1} sum_q w_A^{q k1} x[B q + b].
+//  * line_op: y = F_{-S}( tab . F_S(x) ) along lines held in LDS, with the
+//    two-level split N = N1*N2 across threads.  Phase 1 (N1-point DFTs and
+//    inter-level twiddles), phase 2 (N2-point DFT, table multiply, inverse
+//    N2-point DFT, inverse twiddles) and phase 1' (inverse N1-point DFTs)
+//    each read and write the same LDS slots, so the whole operator costs
+//    two LDS exchanges and three barriers, and the spectrum never needs to
+//    be put back in natural order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include <utility>
+
+namespace opo {
+
+template <typename T> struct alignas(2 * sizeof(T)) cplx {
+    T x, y;
+};
+
+template <typename T> __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+    return {a.x + b.x, a.y + b.y};
+}
+template <typename T> __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+    return {a.x - b.x, a.y - b.y};
+}
+// (a.x b.x - a.y b.y, a.x b.y + a.y b.x): the product of Operators.cuh:109-116.
+template <typename T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T> __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T> __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) {
+    return {a.x * s, a.y * s};
+}
+template <typename T> __device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return {a.x, -a.y}; }
+// (S*i) * a
+template <int S, typename T> __device__ __forceinline__ cplx<T> mul_si(cplx<T> a) {
+    return S > 0 ? cplx<T>{-a.y, a.x} : cplx<T>{a.y, -a.x};
+}
+
+// Twiddles and diagonal tables are applied as a * (hi + lo), where hi is the
+// constant rounded to T and lo the rounding residual (FP32 only).  A plain
+// FP32 twiddle has |w|^2 != 1 by up to ~6e-8, so a forward/inverse pair
+// scales the field by a fixed factor; that factor is the same on every
+// z-slice and accumulates linearly over the S slices.  The small product
+// a*lo is folded into the FMA chain ahead of the exact products a*hi, so
+// it survives their rounding (added after rounding, a sub-half-ulp term is
+// simply lost).  For T = double lo is zero and skipped.
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul_hl(cplx<T> a, cplx<T> hi, cplx<T> lo) {
+    if constexpr (sizeof(T) == 8) {
+        return cmul(a, hi);
+    } else {
+        const T cr = a.x * lo.x - a.y * lo.y, ci = a.x * lo.y + a.y * lo.x;
+        return {fmaf(a.x, hi.x, fmaf(-a.y, hi.y, cr)), fmaf(a.x, hi.y, fmaf(a.y, hi.x, ci))};
+    }
+}
+// a * conj(hi + lo)
+template <typename T>
+__device__ __forceinline__ cplx<T> cmulc_hl(cplx<T> a, cplx<T> hi, cplx<T> lo) {
+    if constexpr (sizeof(T) == 8) {
+        return cmulc(a, hi);
+    } else {
+        const T cr = a.x * lo.x + a.y * lo.y, ci = a.y * lo.x - a.x * lo.y;
+        return {fmaf(a.x, hi.x, fmaf(a.y, hi.y, cr)), fmaf(a.y, hi.x, fmaf(-a.x, hi.y, ci))};
+    }
+}
+
+// ------------------------------------------------- compile-time twiddles
+namespace ct {
+constexpr double kPi4 = 0.78539816339744830961566084581987572;  // pi/4
+
+constexpr double taylor_sin(double x) {
+    double x2 = x * x, term = x, sum = x;
+    for (int k = 1; k < 14; ++k) {
+        term *= -x2 / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_cos(double x) {
+    double x2 = x * x, term = 1.0, sum = 1.0;
+    for (int k = 1; k < 14; ++k) {
+        term *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+// cos and sin of 2*pi*j/n with the octant found in exact integer arithmetic.
+constexpr void cis2pi(long long j, long long n, double &c, double &s) {
+    j %= n;
+    if (j < 0) j += n;
+    long long m = 8 * j;
+    long long o = m / n;
+    long long r = m - o * n;                   // fraction r/n of the octant
+    double x = kPi4 * ((double)r / (double)n);
+    double xp = kPi4 * ((double)(n - r) / (double)n);
+    double C = taylor_cos(x), S = taylor_sin(x), Cp = taylor_cos(xp), Sp = taylor_sin(xp);
+    switch (o) {
+        case 0: c = C; s = S; break;
+        case 1: c = Sp; s = Cp; break;
+        case 2: c = -S; s = C; break;
+        case 3: c = -Cp; s = Sp; break;
+        case 4: c = -C; s = -S; break;
+        case 5: c = -Sp; s = -Cp; break;
+        case 6: c = S; s = -C; break;
+        default: c = Cp; s = -Sp; break;
+    }
+    if (r == 0) {  // exact multiples of pi/4 where a component is exactly 0 or +-1
+        if (o == 0) { c = 1.0; s = 0.0; }
+        if (o == 2) { c = 0.0; s = 1.0; }
+        if (o == 4) { c = -1.0; s = 0.0; }
+        if (o == 6) { c = 0.0; s = -1.0; }
+    }
+}
+constexpr double cos2pi(long long j, long long n) {
+    double c = 0, s = 0;
+    cis2pi(j, n, c, s);
+    return c;
+}
+constexpr double sin2pi(long long j, long long n) {
+    double c = 0, s = 0;
+    cis2pi(j, n, c, s);
+    return s;
+}
+}  // namespace ct
+
+template <int V> using ic = std::integral_constant<int, V>;
+
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(ic<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// a * exp(i * 2*pi * J / N), J and N compile-time, with the trivial cases
+// (+-1, +-i) done by swaps so no rounding is introduced.
+template <int N, int J, typename T> __device__ __forceinline__ cplx<T> twiddle_ct(cplx<T> a) {
+    constexpr int j = ((J % N) + N) % N;
+    if constexpr (j == 0) {
+        return a;
+    } else if constexpr (4 * j == N) {
+        return mul_si<+1>(a);
+    } else if constexpr (2 * j == N) {
+        return cplx<T>{-a.x, -a.y};
+    } else if constexpr (4 * j == 3 * N) {
+        return mul_si<-1>(a);
+    } else {
+        constexpr double c = ct::cos2pi(j, N), s = ct::sin2pi(j, N);
+        constexpr T ch = (T)c, sh = (T)s;
+        constexpr T cl = (T)(c - (double)ch), sl = (T)(s - (double)sh);
+        return cmul_hl(a, cplx<T>{ch, sh}, cplx<T>{cl, sl});
+    }
+}
+
+// ------------------------------------------------------ register DFTs
+template <int N> constexpr int first_factor() {
+    if constexpr (N % 4 == 0 && N > 4) return 4;
+    else if constexpr (N % 2 == 0 && N > 2) return 2;
+    else if constexpr (N % 3 == 0 && N > 3) return 3;
+    else return N;
+}
+
+template <int N, int S, typename T> __device__ __forceinline__ void dft(cplx<T> *v);
+
+template <int S, typename T> __device__ __forceinline__ void dft2(cplx<T> *v) {
+    cplx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+template <int S, typename T> __device__ __forceinline__ void dft3(cplx<T> *v) {
+    constexpr double r3 = 0.86602540378443864676372317075294;  // sqrt(3)/2
+    constexpr T r3h = (T)r3, r3l = (T)(r3 - (double)r3h);
+    cplx<T> a0 = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    cplx<T> u = {a0.x - (T)0.5 * t1.x, a0.y - (T)0.5 * t1.y};
+    cplx<T> w = mul_si<S>(cmul_hl(t2, cplx<T>{r3h, (T)0}, cplx<T>{r3l, (T)0}));
+    v[0] = cadd(a0, t1);
+    v[1] = cadd(u, w);
+    v[2] = csub(u, w);
+}
+template <int S, typename T> __device__ __forceinline__ void dft4(cplx<T> *v) {
+    cplx<T> t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    cplx<T> t2 = cadd(v[1], v[3]), t3 = mul_si<S>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[2] = csub(t0, t2);
+    v[3] = csub(t1, t3);
+}
+
+template <int N, int S, typename T> __device__ __forceinline__ void dft(cplx<T> *v) {
+    if constexpr (N == 1) {
+        return;
+    } else if constexpr (N == 2) {
+        dft2<S>(v);
+    } else if constexpr (N == 3) {
+        dft3<S>(v);
+    } else if constexpr (N == 4) {
+        dft4<S>(v);
+    } else {
+        constexpr int A = first_factor<N>();
+        constexpr int B = N / A;
+        static_assert(A > 1 && B > 1, "unsupported register DFT length");
+        cplx<T> y[N];
+        static_for<0, B>([&](auto b_) {
+            constexpr int b = decltype(b_)::value;
+            cplx<T> a[A];
+            static_for<0, A>([&](auto q_) {
+                constexpr int q = decltype(q_)::value;
+                a[q] = v[B * q + b];
+            });
+            dft<A, S>(a);
+            static_for<0, A>([&](auto k_) {
+                constexpr int k1 = decltype(k_)::value;
+                y[k1 * B + b] = twiddle_ct<N, S * b * k1>(a[k1]);
+            });
+        });
+        static_for<0, A>([&](auto k_) {
+            constexpr int k1 = decltype(k_)::value;
+            cplx<T> c[B];
+            static_for<0, B>([&](auto b_) {
+                constexpr int b = decltype(b_)::value;
+                c[b] = y[k1 * B + b];
+            });
+            dft<B, S>(c);
+            static_for<0, B>([&](auto k2_) {
+                constexpr int k2 = decltype(k2_)::value;
+                v[k1 + A * k2] = c[k2];
+            });
+        });
+    }
+}
+
+// ----------------------------------------------------- line operator
+// Two-level split of each line length N into N1 * N2 (N1, N2 <= 32).
+template <int N> struct Split {
+    static constexpr int N1 = (N <= 32) ? N : (N == 64) ? 8 : (N == 96) ? 12 : (N == 128) ? 16
+                            : (N == 256) ? 16 : (N == 512) ? 32 : (N == 1024) ? 32 : (N == 2048) ? 64 : 0;
+    static constexpr int N2 = N / N1;
+    static_assert(N1 * N2 == N, "unsupported line length");
+};
+
+// A batch of lines in LDS: element n of line l is at lds[l*LS + n*ES].
+// tw: W_N^j = exp(-2 pi i j / N), j < N, as (hi, lo) pairs tw[2j], tw[2j+1]
+// (FP64-built).  tab: the diagonal operator in the bin order of the first
+// transform, already holding any normalisation, also as (hi, lo) pairs.
+// S is the exponent sign of the first
+// transform: +1 for the time axis (inverse first, Solver3D.cuh:606-626),
+// -1 for x and y (forward first, Solver3D.cuh:757-777).
+// Threads map line-fastest, so consecutive lanes touch consecutive lines.
+template <typename T, int N, int S, int LS, int ES>
+__device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> *__restrict__ tw,
+                                        const cplx<T> *__restrict__ tab, int tid, int nthr) {
+    constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+    if constexpr (N2 == 1) {
+        for (int l = tid; l < nlines; l += nthr) {
+            cplx<T> v[N];
+            cplx<T> *base = lds + l * LS;
+#pragma unroll
+            for (int n = 0; n < N; ++n) v[n] = base[n * ES];
+            dft<N, S>(v);
+#pragma unroll
+            for (int k = 0; k < N; ++k) v[k] = cmul_hl(v[k], tab[2 * k], tab[2 * k + 1]);
+            dft<N, -S>(v);
+#pragma unroll
+            for (int n = 0; n < N; ++n) base[n * ES] = v[n];
+        }
+        __syncthreads();
+    } else {
+        // phase 1: task (l, n2)
+        for (int task = tid; task < nlines * N2; task += nthr) {
+            const int l = task % nlines, n2 = task / nlines;
+            cplx<T> *base = lds + l * LS + n2 * ES;
+            cplx<T> v[N1];
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[n1 * N2 * ES];
+            dft<N1, S>(v);
+#pragma unroll
+            for (int k1 = 1; k1 < N1; ++k1) {
+                const int j = n2 * k1;
+                v[k1] = S < 0 ? cmul_hl(v[k1], tw[2 * j], tw[2 * j + 1])
+                              : cmulc_hl(v[k1], tw[2 * j], tw[2 * j + 1]);
+            }
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) base[k1 * N2 * ES] = v[k1];
+        }
+        __syncthreads();
+        // phase 2: task (l, k1)
+        for (int task = tid; task < nlines * N1; task += nthr) {
+            const int l = task % nlines, k1 = task / nlines;
+            cplx<T> *base = lds + l * LS + k1 * N2 * ES;
+            cplx<T> v[N2];
+#pragma unroll
+            for (int n2 = 0; n2 < N2; ++n2) v[n2] = base[n2 * ES];
+            dft<N2, S>(v);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const int k = k1 + N1 * k2;
+                v[k2] = cmul_hl(v[k2], tab[2 * k], tab[2 * k + 1]);
+            }
+            dft<N2, -S>(v);
+#pragma unroll
+            for (int n2 = 1; n2 < N2; ++n2) {
+                const int j = n2 * k1;
+                v[n2] = S < 0 ? cmulc_hl(v[n2], tw[2 * j], tw[2 * j + 1])
+                              : cmul_hl(v[n2], tw[2 * j], tw[2 * j + 1]);
+            }
+#pragma unroll
+            for (int n2 = 0; n2 < N2; ++n2) base[n2 * ES] = v[n2];
+        }
+        __syncthreads();
+        // phase 1': task (l, n2)
+        for (int task = tid; task < nlines * N2; task += nthr) {
+            const int l = task % nlines, n2 = task / nlines;
+            cplx<T> *base = lds + l * LS + n2 * ES;
+            cplx<T> v[N1];
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[k1 * N2 * ES];
+            dft<N1, -S>(v);
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) base[n1 * N2 * ES] = v[n1];
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace opo

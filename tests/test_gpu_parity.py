@@ -1,0 +1,147 @@
+"""GPU path against the FP64 oracle, through the C ABI.
+
+Each operator is checked on its own (random fields exercise every Fourier
+bin), then whole single passes and round trips.  Tolerances: one operator
+1e-6 relative L2 (FP32 bulk); a whole single pass 1e-5 per field; per-trip
+power 1e-6 relative; the FP64 device path 1e-11.
+"""
+import numpy as np
+import pytest
+
+import cuopo3d_amd as co
+from cuopo3d_amd import configs
+from cuopo3d_amd.capi import Grid, OPO_FP32, OPO_FP64
+from oracle.pyoracle import Oracle
+
+from helpers import rel_l2, random_fields, cavity_params
+
+pytestmark = pytest.mark.gpu
+
+OP_GRIDS = [(16, 16, 16), (32, 32, 64), (64, 64, 32), (96, 96, 16), (128, 128, 16),
+            (16, 16, 128), (16, 16, 256), (16, 16, 512), (16, 16, 1024), (1, 1, 64)]
+
+
+def _pair(grid, p, fields):
+    s = co.Solver(grid, p, device=0)
+    s.set_fields(*fields)
+    o = Oracle(grid, p)
+    o.set_fields(*fields)
+    return s, o
+
+
+def _check(s, o, tol, what):
+    g = s.get_fields()
+    r = o.get_fields()
+    errs = [rel_l2(a, b) for a, b in zip(g, r)]
+    print(f"{what}: rel L2 p/s/i = " + " ".join(f"{e:.2e}" for e in errs), flush=True)
+    for e in errs:
+        assert e < tol, (what, errs)
+
+
+@pytest.mark.parametrize("n", OP_GRIDS, ids=lambda n: "x".join(map(str, n)))
+def test_dispersion_operator(n, require_gpu):
+    grid, p, _ = cavity_params(n, lcr=5000.0)
+    # exaggerate dispersion so every bin carries a large, distinct phase
+    for f in range(3):
+        p.beta2[f] *= 200.0
+        p.delta[f] += 0.01 * (f + 1)
+    s, o = _pair(grid, p, random_fields(grid, 11))
+    s.dispersion()
+    o.dispersion()
+    _check(s, o, 1e-6, f"D {n}")
+
+
+@pytest.mark.parametrize("n", OP_GRIDS, ids=lambda n: "x".join(map(str, n)))
+def test_diffraction_operator(n, require_gpu):
+    grid, p, _ = cavity_params(n)
+    s, o = _pair(grid, p, random_fields(grid, 12))
+    s.diffractionInCrystal()
+    o.diffraction()
+    _check(s, o, 1e-6, f"Q {n}")
+
+
+@pytest.mark.parametrize("n", [(32, 32, 64), (128, 128, 16)], ids=lambda n: "x".join(map(str, n)))
+def test_free_space_and_gdd_and_mirror(n, require_gpu):
+    grid, p, _ = cavity_params(n, rs=0.9, phase_s=0.7)
+    s, o = _pair(grid, p, random_fields(grid, 13))
+    s.diffractionInFreeSpace()
+    o.free_space()
+    _check(s, o, 1e-6, f"Qfs {n}")
+    s.GDDCompensation()
+    o.gdd()
+    _check(s, o, 1e-6, f"GDD {n}")
+    s.applyReflectivity()
+    o.mirror()
+    _check(s, o, 1e-6, f"mirror {n}")
+
+
+@pytest.mark.parametrize("h_in_stages", [0, 1])
+def test_nonlinear_operator(h_in_stages, require_gpu):
+    grid, p, _ = cavity_params((32, 32, 64))
+    p.rk4_h_in_stages = h_in_stages
+    # strong coupling so the step changes the fields by ~1 %
+    for f in range(3):
+        p.kappa[f] = 4e-4 * (1 + 0.5 * f)
+    fields = random_fields(grid, 14, amp=(1.0, 0.5, 0.5))
+    s, o = _pair(grid, p, fields)
+    s.solverRK4()
+    o.nonlinear()
+    g, r = s.get_fields(), o.get_fields()
+    for a, b, x in zip(g, r, fields):
+        d_gpu, d_ref = a - x, b - x
+        e = rel_l2(d_gpu, d_ref)
+        print(f"N: rel L2 of the increment {e:.2e}", flush=True)
+        assert e < 2e-5
+        assert rel_l2(a, b) < 1e-6
+
+
+def test_single_pass_32(require_gpu):
+    grid, p, ph, fields = configs.single_pass_32()
+    s, o = _pair(grid, p, fields)
+    s.runSinglePass()
+    o.ssfm()
+    _check(s, o, 1e-5, "single pass 32x32x64")
+
+
+def test_round_trips_small_cavity(require_gpu):
+    grid, p, ph, fields = configs.cavity_64(n=(32, 32, 64), n_round_trips=4)
+    s, o = _pair(grid, p, fields)
+    pg = s.runOPO(4)
+    pr = o.run_round_trips(4)
+    print("power gpu", pg[:, 1], "oracle", pr[:, 1], flush=True)
+    _check(s, o, 1e-5, "4 trips 32x32x64")
+    rel = np.abs(pg - pr) / np.abs(pr)
+    assert rel.max() < 1e-6, rel
+
+
+def test_fp64_single_pass(require_gpu):
+    grid, p, ph, fields = configs.single_pass_32(precision=OPO_FP64)
+    s, o = _pair(grid, p, fields)
+    s.runSinglePass()
+    o.ssfm()
+    _check(s, o, 1e-11, "fp64 single pass 32x32x64")
+
+
+def test_cavity_64_first_trips(require_gpu):
+    """64x64x256 cavity workload: first 3 round trips, fields and per-trip power."""
+    grid, p, ph, fields = configs.cavity_64(n_round_trips=200)
+    s, o = _pair(grid, p, fields)
+    pg = s.runOPO(3)
+    pr = o.run_round_trips(3)
+    rel = np.abs(pg - pr) / np.abs(pr)
+    print("per-trip power rel err (p,s,i):", rel.max(axis=0), flush=True)
+    _check(s, o, 1e-5, "cavity 64x64x256, 3 trips")
+    assert rel.max() < 1e-6, rel
+
+
+def test_femto_single_pass(require_gpu):
+    """64x64x1024, 150 fs sech^2 pump, 399 slices."""
+    grid, p, ph, fields = configs.femto_64()
+    assert p.n_slices == 399
+    s, o = _pair(grid, p, fields)
+    s.runSinglePass()
+    o.ssfm()
+    _check(s, o, 1e-5, "femto 64x64x1024 single pass")
+    pg, pr = s.averagePowerCutXY(), o.power()
+    print("power rel err", np.abs(pg - pr) / pr, flush=True)
+    assert np.all(np.abs(pg - pr) / pr < 1e-6)
