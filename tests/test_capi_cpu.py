@@ -1,0 +1,71 @@
+"""The C-ABI library on a host without a GPU: it loads, exports every
+function the headers declare, and rejects bad input with error codes."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from cuopo3d_amd import capi, configs
+from cuopo3d_amd.capi import Grid, OPO_ERR_ARG, OPO_ERR_NOGPU, OPO_ERR_UNSUPPORTED
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(opo_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    lib = C.CDLL(capi.LIB_PATH)
+    names = declared()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_symbol():
+    lib = capi.load()   # binds argtypes for every name it knows
+    for n in declared():
+        assert getattr(lib, n).argtypes is not None, n
+
+
+def test_create_rejects_bad_input():
+    lib = capi.load()
+    g, p, _, _ = configs.single_pass_32()
+    h = C.c_void_p()
+    assert lib.opo_create(C.byref(Grid(0, 32, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_ARG
+    assert lib.opo_create(C.byref(Grid(100, 100, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_UNSUPPORTED
+    assert b"not supported" in lib.opo_last_error()
+    assert lib.opo_create(C.byref(Grid(32, 64, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_UNSUPPORTED
+    bad = p.copy()
+    bad.precision = 7
+    assert lib.opo_create(C.byref(g), C.byref(bad), 0, C.byref(h)) == OPO_ERR_ARG
+    bad = p.copy()
+    bad.dz = 0.0
+    assert lib.opo_create(C.byref(g), C.byref(bad), 0, C.byref(h)) == OPO_ERR_ARG
+    if lib.opo_device_count() == 0:
+        assert lib.opo_create(C.byref(g), C.byref(p), 0, C.byref(h)) == OPO_ERR_NOGPU
+    assert not h.value
+
+
+def test_null_handles_are_errors():
+    lib = capi.load()
+    assert lib.opo_ssfm(None) == OPO_ERR_ARG
+    assert lib.opo_run_round_trips(None, 1, None) == OPO_ERR_ARG
+    assert lib.opo_batch_step(None, None) == OPO_ERR_ARG
+    assert lib.opo_field_bytes(None) == 0
+    lib.opo_destroy(None)
+
+
+def test_batch_footprint():
+    lib = capi.load()
+    g = Grid(128, 128, 1024)
+    fp = lib.opo_batch_footprint(C.byref(g), 0)
+    assert 4 * 8 * 128 * 128 * 1024 <= fp < 4 * 8 * 128 * 128 * 1024 * 1.01
