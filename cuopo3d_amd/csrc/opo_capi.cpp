@@ -376,8 +376,25 @@ template <typename T> struct SolveT final : SolveImpl {
         return OPO_OK;
     }
 
-    // SSFM (Solver3D.cuh:946-960) on the fused schedule.  trip: read the pump
-    // from Api and apply the mirror factors at the end.
+    // Fused pass: pointwise half step(s) then the along-t operator on fields
+    // first..2 with tables tab0 + (f - first) * tab_stride.
+    int q_nl_time(const C *psrc, const Rk4Args<T> &rk, const C *tab0, int tab_stride, int first,
+                  int stage, const int *st) {
+        TLine3Args<T> a{};
+        a.psrc = psrc;
+        for (int f = 0; f < 3; ++f) a.f[f] = A[f];
+        a.tab0 = tab0;
+        a.tab_stride = tab_stride;
+        a.first_field = first;
+        a.rk = rk;
+        PROF(stage, (launch_tline3<T>(stream, geo(), a, tw_t, st)));
+        return OPO_OK;
+    }
+
+    // SSFM (Solver3D.cuh:946-960) on the fused schedule: per slice one pass
+    // [N(h) of slice k-1, N(h) of slice k, D] and one pass [Q].  trip: read
+    // the pump from Api, and end with [N(h), mirror, GDD of s and i] (the
+    // mirror and GDD commute with the free-space step that follows).
     int q_ssfm(bool trip, const int *st) {
         const int S = p.n_slices;
         int rc;
@@ -387,21 +404,22 @@ template <typename T> struct SolveT final : SolveImpl {
         }
         for (int k = 0; k < S; ++k) {
             const double z = k * p.dz;
-            if (k == 0) rc = q_rk4(trip ? Api : A[0], rk_args(1, z, 0.0, false), st);
-            else rc = q_rk4(A[0], rk_args(2, (k - 1) * p.dz, z, false), st);
-            if (rc) return rc;
-            if ((rc = q_dispersion(st))) return rc;
+            const C *psrc = (k == 0 && trip) ? Api : A[0];
+            Rk4Args<T> rk = k == 0 ? rk_args(1, z, 0.0, false) : rk_args(2, (k - 1) * p.dz, z, false);
+            if ((rc = q_nl_time(psrc, rk, tabT[0], (int)(tabT[1] - tabT[0]), 0, OPO_STAGE_TIME_OP, st)))
+                return rc;
             if ((rc = q_diffraction(st))) return rc;
         }
-        return q_rk4(A[0], rk_args(1, (S - 1) * p.dz, 0.0, trip), st);
+        const Rk4Args<T> last = rk_args(1, (S - 1) * p.dz, 0.0, trip);
+        if (trip && p.gdd_on) return q_nl_time(A[0], last, tabG, 0, 1, OPO_STAGE_GDD, st);
+        return q_rk4(A[0], last, st);
     }
 
     // One round trip (Solver3D.cuh:979-1011) with the per-trip power.
     int q_trip(bool fs) {
-        int rc = q_ssfm(true, flags);
+        int rc = q_ssfm(true, flags);  // ends with the mirror and GDD
         if (rc) return rc;
         if (fs && (rc = q_free_space(flags))) return rc;
-        if (p.gdd_on && (rc = q_gdd(flags))) return rc;
         int prc = prof_mark(OPO_STAGE_POWER);
         if (prc) return prc;
         CK(launch_power_partial<T>(stream, A[0], A[1], A[2], N, nblk, partial, flags));

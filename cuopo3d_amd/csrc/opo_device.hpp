@@ -259,14 +259,31 @@ template <int N> struct Split {
 // transform: +1 for the time axis (inverse first, Solver3D.cuh:606-626),
 // -1 for x and y (forward first, Solver3D.cuh:757-777).
 // Threads map line-fastest, so consecutive lanes touch consecutive lines.
+// Up to three fields share each phase (and its barrier): field f's lines
+// start at b0 + f*bstride in LDS and its table at t0 + f*tstride; tasks
+// enumerate (line, sub-index, field) line-fastest.  Strides rather than
+// pointer arrays keep the per-field addresses in registers (an indexed
+// array, or selects between pointers, end up in scratch memory).
+template <typename T> struct LineSet {
+    cplx<T> *b0;
+    int bstride;
+    const cplx<T> *t0;
+    int tstride;
+    int nf;
+    __device__ __forceinline__ cplx<T> *base(int f) const { return b0 + f * bstride; }
+    __device__ __forceinline__ const cplx<T> *tab(int f) const { return t0 + f * tstride; }
+};
+
 template <typename T, int N, int S, int LS, int ES>
-__device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> *__restrict__ tw,
-                                        const cplx<T> *__restrict__ tab, int tid, int nthr) {
+__device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
+                                            const cplx<T> *__restrict__ tw, int tid, int nthr) {
     constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
     if constexpr (N2 == 1) {
-        for (int l = tid; l < nlines; l += nthr) {
+        for (int task = tid; task < nlines * ls.nf; task += nthr) {
+            const int l = task % nlines, f = task / nlines;
+            const cplx<T> *tab = ls.tab(f);
             cplx<T> v[N];
-            cplx<T> *base = lds + l * LS;
+            cplx<T> *base = ls.base(f) + l * LS;
 #pragma unroll
             for (int n = 0; n < N; ++n) v[n] = base[n * ES];
             dft<N, S>(v);
@@ -278,10 +295,10 @@ __device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> 
         }
         __syncthreads();
     } else {
-        // phase 1: task (l, n2)
-        for (int task = tid; task < nlines * N2; task += nthr) {
-            const int l = task % nlines, n2 = task / nlines;
-            cplx<T> *base = lds + l * LS + n2 * ES;
+        // phase 1: task (l, n2, f)
+        for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
+            const int l = task % nlines, r = task / nlines, n2 = r % N2, f = r / N2;
+            cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
             cplx<T> v[N1];
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[n1 * N2 * ES];
@@ -296,10 +313,11 @@ __device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> 
             for (int k1 = 0; k1 < N1; ++k1) base[k1 * N2 * ES] = v[k1];
         }
         __syncthreads();
-        // phase 2: task (l, k1)
-        for (int task = tid; task < nlines * N1; task += nthr) {
-            const int l = task % nlines, k1 = task / nlines;
-            cplx<T> *base = lds + l * LS + k1 * N2 * ES;
+        // phase 2: task (l, k1, f)
+        for (int task = tid; task < nlines * N1 * ls.nf; task += nthr) {
+            const int l = task % nlines, r = task / nlines, k1 = r % N1, f = r / N1;
+            const cplx<T> *tab = ls.tab(f);
+            cplx<T> *base = ls.base(f) + l * LS + k1 * N2 * ES;
             cplx<T> v[N2];
 #pragma unroll
             for (int n2 = 0; n2 < N2; ++n2) v[n2] = base[n2 * ES];
@@ -320,10 +338,10 @@ __device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> 
             for (int n2 = 0; n2 < N2; ++n2) base[n2 * ES] = v[n2];
         }
         __syncthreads();
-        // phase 1': task (l, n2)
-        for (int task = tid; task < nlines * N2; task += nthr) {
-            const int l = task % nlines, n2 = task / nlines;
-            cplx<T> *base = lds + l * LS + n2 * ES;
+        // phase 1': task (l, n2, f)
+        for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
+            const int l = task % nlines, r = task / nlines, n2 = r % N2, f = r / N2;
+            cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
             cplx<T> v[N1];
 #pragma unroll
             for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[k1 * N2 * ES];
@@ -333,6 +351,14 @@ __device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> 
         }
         __syncthreads();
     }
+}
+
+// One field.
+template <typename T, int N, int S, int LS, int ES>
+__device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> *__restrict__ tw,
+                                        const cplx<T> *__restrict__ tab, int tid, int nthr) {
+    LineSet<T> ls{lds, 0, tab, 0, 1};
+    line_op_set<T, N, S, LS, ES>(ls, nlines, tw, tid, nthr);
 }
 
 }  // namespace opo

@@ -33,6 +33,24 @@ struct Geometry {
     int nx, ny, nt;
 };
 
+// Fused pass: nonlinear half step(s) of all three fields followed by the
+// along-t operator on fields first_field..2, per tile of pixels (field f
+// uses the table tab0 + (f - first_field)*tab_stride).  Fields below
+// first_field are stored right after the pointwise step.  rk.nsteps == 0
+// skips the pointwise step.
+template <typename T> struct TLine3Args {
+    const cplx<T> *psrc;   // pump read from here (Api on a trip's first slice)
+    cplx<T> *f[3];         // Ap, As, Ai
+    const cplx<T> *tab0;
+    int tab_stride;
+    int first_field;       // 0: all three fields; 1: signal and idler
+    Rk4Args<T> rk;
+};
+
+template <typename T>
+hipError_t launch_tline3(hipStream_t st, const Geometry &g, const TLine3Args<T> &a,
+                         const cplx<T> *tw_t, const int *status);
+
 // Launchers; return hipError_t of the launch.
 template <typename T>
 hipError_t launch_rk4(hipStream_t st, const cplx<T> *psrc, cplx<T> *ap, cplx<T> *as, cplx<T> *ai,

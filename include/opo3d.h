@@ -183,11 +183,12 @@ int opo_sync(opo_solve *s);
  * HIP events on the solve's stream, and returns per stage kind the summed
  * kernel time (ms) and the launch count.  The fields advance as in
  * opo_run_round_trips. */
-#define OPO_STAGE_NONLINEAR  0  /* pointwise RK4 half step(s), all fields  */
-#define OPO_STAGE_TIME_OP    1  /* along-t operator D on all fields          */
+#define OPO_STAGE_NONLINEAR  0  /* stand-alone pointwise RK4 half step     */
+#define OPO_STAGE_TIME_OP    1  /* RK4 half steps + along-t operator D,
+                                   all fields, one pass                      */
 #define OPO_STAGE_XY_OP      2  /* xy operator Q on all fields               */
 #define OPO_STAGE_FREE_SPACE 3  /* free-space xy operator on s, i            */
-#define OPO_STAGE_GDD        4  /* GDD along-t operator on s, i              */
+#define OPO_STAGE_GDD        4  /* last RK4 half step + mirror + GDD on s, i */
 #define OPO_STAGE_POWER      5  /* per-trip power reduction                  */
 #define OPO_N_STAGES         6
 int opo_profile_trips(opo_solve *s, int n, double ms[OPO_N_STAGES],
