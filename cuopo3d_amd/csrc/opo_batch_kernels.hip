@@ -1,0 +1,237 @@
+// opo_batch_kernels.hip -- split-step passes over many solves at once.
+//
+// A batch holds solves of different grids resident in HBM.  Each pass is
+// one launch per size class (the NT of the along-t pass, the NX of the xy
+// pass) whose workgroups walk a work list of (solve, tile) items, so small
+// solves share the GPU instead of each under-filling it.  Every solve's
+// arithmetic (tiles, line operators, reduction order) is the same as when it
+// runs alone, so its results do not depend on its neighbours.
+#include "opo_device.hpp"
+#include "opo_kernels.hpp"
+#include "opo_passes.hpp"
+#include "opo_rk4.hpp"
+
+namespace opo {
+
+template <typename T>
+__device__ __forceinline__ Rk4Args<T> batch_rk(const SolveDesc<T> &d, int mode) {
+    Rk4Args<T> r{};
+    r.kp = d.kp;
+    r.ks = d.ks;
+    r.ki = d.ki;
+    r.c1 = d.c1;
+    r.c3 = d.c3;
+    r.h = d.h;
+    r.nsteps = mode == BATCH_MID ? 2 : 1;
+    r.has_phase = 0;
+    r.out_scale = mode == BATCH_LAST ? 1 : 0;
+    r.ms_re = d.ms_re;
+    r.ms_im = d.ms_im;
+    r.ms_lo_re = d.ms_lo_re;
+    r.ms_lo_im = d.ms_lo_im;
+    r.mi_re = d.mi_re;
+    r.mi_im = d.mi_im;
+    r.mi_lo_re = d.mi_lo_re;
+    r.mi_lo_im = d.mi_lo_im;
+    return r;
+}
+
+template <typename T, int NT, int TX, int BS>
+__global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ sd,
+                                              const int *__restrict__ status,
+                                              const WorkItem *__restrict__ work, int mode) {
+    __shared__ cplx<T> lds[3 * NT * TX];
+    const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
+    if (status[w.solve]) return;
+    const SolveDesc<T> &d = sd[w.solve];
+    TLine3Args<T> a;
+    a.psrc = mode == BATCH_FIRST ? d.api : d.f[0];
+    a.f[0] = d.f[0];
+    a.f[1] = d.f[1];
+    a.f[2] = d.f[2];
+    if (mode == BATCH_LAST) {
+        a.tab0 = d.tabG;
+        a.tab_stride = 0;
+        a.first_field = d.gdd_on ? 1 : 3;
+    } else {
+        a.tab0 = d.tabT;
+        a.tab_stride = 2 * NT;
+        a.first_field = 0;
+    }
+    a.rk = batch_rk(d, mode);
+    nl_time_tile<T, NT, TX, BS, false>(a, d.nx, d.ny, w.item, d.twt, lds);
+}
+
+template <typename T, int NX, int BS>
+__global__ void __launch_bounds__(BS) kb_xy(const SolveDesc<T> *__restrict__ sd,
+                                            const TripState *__restrict__ ts,
+                                            const int *__restrict__ status,
+                                            const WorkItem *__restrict__ work, int free_space) {
+    __shared__ cplx<T> lds[NX * (NX + 1)];
+    const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
+    if (status[w.solve]) return;
+    if (free_space && !ts[w.solve].fs) return;
+    const SolveDesc<T> &d = sd[w.solve];
+    const int k = w.item / d.nt, t = w.item % d.nt;
+    const int f = free_space ? 1 + k : k;
+    const cplx<T> *tx = free_space ? d.qfsx + 2 * NX * k : d.qx + 2 * NX * k;
+    const cplx<T> *ty = free_space ? d.qfsy + 2 * NX * k : d.qy + 2 * NX * k;
+    cplx<T> *A = (f == 0 ? d.f[0] : (f == 1 ? d.f[1] : d.f[2])) + (size_t)t * NX * NX;
+    xy_plane<T, NX, BS>(A, tx, ty, d.twx, lds);
+}
+
+__device__ __forceinline__ void neumaier(double &sum, double &comp, double v) {
+    double t = sum + v;
+    if (fabs(sum) >= fabs(v)) comp += (sum - t) + v;
+    else comp += (v - t) + sum;
+    sum = t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_power_partial(const SolveDesc<T> *__restrict__ sd,
+                                                        const int *__restrict__ status,
+                                                        const WorkItem *__restrict__ work,
+                                                        double *partial, int max_chunk) {
+    __shared__ double red[3][256];
+    const WorkItem w = work[blockIdx.x];
+    if (status[w.solve]) return;
+    const SolveDesc<T> &d = sd[w.solve];
+    const size_t n = (size_t)d.nx * d.ny * d.nt;
+    const size_t per = (n + d.nchunk - 1) / d.nchunk;
+    const size_t lo = per * w.item, hi = lo + per < n ? lo + per : n;
+    const int tid = threadIdx.x;
+    for (int f = 0; f < 3; ++f) {
+        const cplx<T> *A = f == 0 ? d.f[0] : (f == 1 ? d.f[1] : d.f[2]);
+        double s = 0.0, c = 0.0;
+        for (size_t i = lo + tid; i < hi; i += 256) {
+            const double re = (double)A[i].x, im = (double)A[i].y;
+            neumaier(s, c, re * re + im * im);
+        }
+        red[f][tid] = s + c;
+    }
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (tid < k)
+            for (int f = 0; f < 3; ++f) red[f][tid] += red[f][tid + k];
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int f = 0; f < 3; ++f)
+            partial[((size_t)w.solve * 3 + f) * max_chunk + w.item] = red[f][0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) kb_power_final(const SolveDesc<T> *__restrict__ sd,
+                                                     const TripState *__restrict__ ts, int *status,
+                                                     const double *partial, int max_chunk) {
+    const int s = blockIdx.x;
+    if (threadIdx.x != 0 || status[s]) return;
+    const SolveDesc<T> &d = sd[s];
+    double pw[3];
+    for (int f = 0; f < 3; ++f) {
+        double sum = 0.0, comp = 0.0;
+        for (int c = 0; c < d.nchunk; ++c) neumaier(sum, comp, partial[((size_t)s * 3 + f) * max_chunk + c]);
+        pw[f] = d.pref[f] * (sum + comp);
+    }
+    const int k = ts[s].trip;
+    if (k < d.series_cap)
+        for (int f = 0; f < 3; ++f) d.series[3 * (size_t)k + f] = pw[f];
+    if (!isfinite(pw[1])) status[s] = k + 1;
+}
+
+int batch_tile_px(int nt, bool fp64) {
+    switch (nt) {
+#define TXC(N) case N: return fp64 ? tline3_tx<double, N>() : tline3_tx<float, N>();
+        TXC(16) TXC(32) TXC(64) TXC(128) TXC(256) TXC(512) TXC(1024)
+#undef TXC
+        default: return 0;
+    }
+}
+
+bool batch_supported(int nx, int ny, int nt, bool fp64) {
+    if (nx != ny) return false;
+    const int tx = batch_tile_px(nt, fp64);
+    if (tx == 0 || nx % tx != 0) return false;
+    switch (nx) {
+        case 16: case 32: case 64: case 96: return true;
+        case 128: return !fp64;
+        default: return false;
+    }
+}
+
+template <typename T, int NT>
+static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const int *status,
+                              const WorkItem *work, int mode) {
+    constexpr int TX = tline3_tx<T, NT>();
+    hipLaunchKernelGGL((kb_time<T, NT, TX, 256>), dim3((unsigned)n_work), dim3(256), 0, st, sd, status,
+                       work, mode);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc<T> *sd,
+                             const int *status, const WorkItem *work, int mode) {
+    if (n_work <= 0) return hipSuccess;
+    switch (nt) {
+        case 16: return bt_dispatch<T, 16>(st, n_work, sd, status, work, mode);
+        case 32: return bt_dispatch<T, 32>(st, n_work, sd, status, work, mode);
+        case 64: return bt_dispatch<T, 64>(st, n_work, sd, status, work, mode);
+        case 128: return bt_dispatch<T, 128>(st, n_work, sd, status, work, mode);
+        case 256: return bt_dispatch<T, 256>(st, n_work, sd, status, work, mode);
+        case 512: return bt_dispatch<T, 512>(st, n_work, sd, status, work, mode);
+        case 1024: return bt_dispatch<T, 1024>(st, n_work, sd, status, work, mode);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T, int NX>
+static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
+                              const int *status, const WorkItem *work, int fs) {
+    constexpr int BS = NX >= 96 ? 512 : 256;
+    hipLaunchKernelGGL((kb_xy<T, NX, BS>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts, status, work, fs);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_xy(hipStream_t st, int nx, int n_work, const SolveDesc<T> *sd,
+                           const TripState *ts, const int *status, const WorkItem *work,
+                           int free_space) {
+    if (n_work <= 0) return hipSuccess;
+    switch (nx) {
+        case 16: return bx_dispatch<T, 16>(st, n_work, sd, ts, status, work, free_space);
+        case 32: return bx_dispatch<T, 32>(st, n_work, sd, ts, status, work, free_space);
+        case 64: return bx_dispatch<T, 64>(st, n_work, sd, ts, status, work, free_space);
+        case 96: return bx_dispatch<T, 96>(st, n_work, sd, ts, status, work, free_space);
+        case 128:
+            if constexpr (sizeof(T) == 4) return bx_dispatch<T, 128>(st, n_work, sd, ts, status, work, free_space);
+            else return hipErrorInvalidValue;
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_batch_power(hipStream_t st, int n_work, int n_solves, const SolveDesc<T> *sd,
+                              const TripState *ts, int *status, const WorkItem *work,
+                              double *partial, int max_chunk) {
+    if (n_work <= 0) return hipSuccess;
+    hipLaunchKernelGGL((kb_power_partial<T>), dim3((unsigned)n_work), dim3(256), 0, st, sd, status, work,
+                       partial, max_chunk);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((kb_power_final<T>), dim3((unsigned)n_solves), dim3(64), 0, st, sd, ts, status,
+                       partial, max_chunk);
+    return hipGetLastError();
+}
+
+#define OPO_BATCH_INST(T)                                                                            \
+    template hipError_t launch_batch_time<T>(hipStream_t, int, int, const SolveDesc<T> *, const int *, \
+                                             const WorkItem *, int);                                 \
+    template hipError_t launch_batch_xy<T>(hipStream_t, int, int, const SolveDesc<T> *,              \
+                                           const TripState *, const int *, const WorkItem *, int);   \
+    template hipError_t launch_batch_power<T>(hipStream_t, int, int, const SolveDesc<T> *,           \
+                                              const TripState *, int *, const WorkItem *, double *,  \
+                                              int);
+OPO_BATCH_INST(float)
+OPO_BATCH_INST(double)
+
+}  // namespace opo

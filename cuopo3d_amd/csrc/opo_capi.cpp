@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <deque>
 #include <map>
 #include <memory>
@@ -122,6 +123,8 @@ struct SolveImpl {
     virtual int finish_trips(int n, double *power) = 0;
     virtual int power(double out[3]) = 0;
     virtual int profile_trips(int n, double *ms, int *launches) = 0;
+    // Write this solve's SolveDesc<T> (T = the solve's precision) to out.
+    virtual void fill_desc(void *out, double *series, int series_cap, int nchunk) const = 0;
     size_t field_bytes() const { return N * (prec == OPO_FP64 ? 16 : 8); }
     int select() {
         CK(hipSetDevice(device));
@@ -587,6 +590,44 @@ template <typename T> struct SolveT final : SolveImpl {
         return OPO_OK;
     }
 
+    void fill_desc(void *out, double *series_out, int cap, int nchunk) const override {
+        SolveDesc<T> d{};
+        d.api = Api;
+        for (int f = 0; f < 3; ++f) d.f[f] = A[f];
+        d.twx = tw_x;
+        d.twt = tw_t;
+        d.tabT = tabT[0];
+        d.tabG = tabG;
+        d.qx = qx[0];
+        d.qy = qy[0];
+        d.qfsx = qfsx[0];
+        d.qfsy = qfsy[0];
+        d.nx = g.nx;
+        d.ny = g.ny;
+        d.nt = g.nt;
+        d.gdd_on = p.gdd_on;
+        const Rk4Args<T> r = rk_args(1, 0.0, 0.0, true);
+        d.kp = r.kp;
+        d.ks = r.ks;
+        d.ki = r.ki;
+        d.c1 = r.c1;
+        d.c3 = r.c3;
+        d.h = r.h;
+        d.ms_re = r.ms_re;
+        d.ms_im = r.ms_im;
+        d.ms_lo_re = r.ms_lo_re;
+        d.ms_lo_im = r.ms_lo_im;
+        d.mi_re = r.mi_re;
+        d.mi_im = r.mi_im;
+        d.mi_lo_re = r.mi_lo_re;
+        d.mi_lo_im = r.mi_lo_im;
+        for (int f = 0; f < 3; ++f) d.pref[f] = p.power_prefactor[f];
+        d.series = series_out;
+        d.series_cap = cap;
+        d.nchunk = nchunk;
+        std::memcpy(out, &d, sizeof(d));
+    }
+
     int power(double out[3]) override {
         int rc = select();
         if (rc) return rc;
@@ -653,14 +694,18 @@ struct opo_solve {
 };
 
 // ----------------------------------------------------------------- batch
+// A batch keeps many solves resident and advances all of them one round trip
+// per opo_batch_step.  Each pass is one launch per size class over a work
+// list of (solve, tile) items; the trip is captured once as a HIP graph and
+// re-captured only when the set of resident solves changes.
 struct BatchEntry {
     uint64_t ticket = 0;
     opo_grid g{};
     opo_params p{};
     std::vector<unsigned char> api, as, ai;  // host copies while queued
     std::unique_ptr<SolveImpl> s;
+    double *d_series = nullptr;
     int n_trips = 0, done_trips = 0;
-    std::vector<double> power;
     size_t footprint = 0;
     bool finished = false;
     int error = OPO_OK;
@@ -676,6 +721,153 @@ struct opo_batch {
     std::vector<BatchEntry *> resident;
     std::map<uint64_t, std::unique_ptr<BatchEntry>> all;
     std::mutex mu;
+    hipStream_t stream = nullptr;
+    // device-side descriptors, rebuilt when the resident set changes
+    void *d_desc = nullptr;
+    int *d_status = nullptr;
+    TripState *d_ts = nullptr;
+    WorkItem *d_work = nullptr;
+    double *d_partial = nullptr;
+    struct Range {
+        int cls, off, n;
+    };
+    std::vector<Range> trange, xyrange, fsrange;
+    Range prange{0, 0, 0};
+    int n_slices = -1, max_chunk = 1;
+    hipGraphExec_t graph = nullptr;
+    bool dirty = true;
+    std::vector<TripState> h_ts;
+    std::vector<int> h_status;
+
+    ~opo_batch() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto &kv : all)
+            if (kv.second->d_series) (void)hipFree(kv.second->d_series);
+        all.clear();
+        (void)hipFree(d_desc);
+        (void)hipFree(d_status);
+        (void)hipFree(d_ts);
+        (void)hipFree(d_work);
+        (void)hipFree(d_partial);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    static int chunks_for(const opo_grid &g) {
+        size_t n = (size_t)g.nx * g.ny * g.nt;
+        size_t c = n / 65536;
+        return (int)(c < 1 ? 1 : (c > 64 ? 64 : c));
+    }
+
+    int rebuild() {
+        if (graph) {
+            (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
+        }
+        (void)hipFree(d_desc);
+        (void)hipFree(d_status);
+        (void)hipFree(d_ts);
+        (void)hipFree(d_work);
+        (void)hipFree(d_partial);
+        d_desc = nullptr;
+        d_status = nullptr;
+        d_ts = nullptr;
+        d_work = nullptr;
+        d_partial = nullptr;
+        const int n = (int)resident.size();
+        const size_t dsz = prec == OPO_FP64 ? sizeof(SolveDesc<double>) : sizeof(SolveDesc<float>);
+        std::vector<unsigned char> hdesc(dsz * n);
+        max_chunk = 1;
+        for (int i = 0; i < n; ++i) max_chunk = std::max(max_chunk, chunks_for(resident[i]->g));
+        for (int i = 0; i < n; ++i) {
+            BatchEntry *e = resident[i];
+            e->s->fill_desc(hdesc.data() + dsz * i, e->d_series, e->n_trips, chunks_for(e->g));
+        }
+        // work lists grouped by size class
+        std::vector<WorkItem> work;
+        trange.clear();
+        xyrange.clear();
+        fsrange.clear();
+        const bool fp64 = prec == OPO_FP64;
+        std::map<int, std::vector<int>> by_nt, by_nx;
+        for (int i = 0; i < n; ++i) {
+            by_nt[resident[i]->g.nt].push_back(i);
+            by_nx[resident[i]->g.nx].push_back(i);
+        }
+        for (auto &kv : by_nt) {
+            Range r{kv.first, (int)work.size(), 0};
+            const int tx = batch_tile_px(kv.first, fp64);
+            for (int i : kv.second) {
+                const opo_grid &g = resident[i]->g;
+                const int tiles = (g.nx / tx) * g.ny;
+                for (int t = 0; t < tiles; ++t) work.push_back(WorkItem{i, t});
+            }
+            r.n = (int)work.size() - r.off;
+            trange.push_back(r);
+        }
+        for (auto &kv : by_nx) {
+            Range r{kv.first, (int)work.size(), 0};
+            for (int i : kv.second)
+                for (int it = 0; it < 3 * resident[i]->g.nt; ++it) work.push_back(WorkItem{i, it});
+            r.n = (int)work.size() - r.off;
+            xyrange.push_back(r);
+            Range q{kv.first, (int)work.size(), 0};
+            for (int i : kv.second)
+                if (resident[i]->p.fs_on)
+                    for (int it = 0; it < 2 * resident[i]->g.nt; ++it) work.push_back(WorkItem{i, it});
+            q.n = (int)work.size() - q.off;
+            if (q.n) fsrange.push_back(q);
+        }
+        prange = Range{0, (int)work.size(), 0};
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < chunks_for(resident[i]->g); ++c) work.push_back(WorkItem{i, c});
+        prange.n = (int)work.size() - prange.off;
+        if (hipMalloc(&d_desc, std::max<size_t>(hdesc.size(), 1)) != hipSuccess ||
+            hipMalloc(&d_status, sizeof(int) * std::max(n, 1)) != hipSuccess ||
+            hipMalloc(&d_ts, sizeof(TripState) * std::max(n, 1)) != hipSuccess ||
+            hipMalloc(&d_work, sizeof(WorkItem) * std::max<size_t>(work.size(), 1)) != hipSuccess ||
+            hipMalloc(&d_partial, sizeof(double) * 3 * (size_t)std::max(n, 1) * max_chunk) != hipSuccess)
+            return fail(OPO_ERR_NOMEM, "batch descriptor allocation failed");
+        CK(hipMemcpyAsync(d_desc, hdesc.data(), hdesc.size(), hipMemcpyHostToDevice, stream));
+        CK(hipMemcpyAsync(d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice, stream));
+        CK(hipMemsetAsync(d_status, 0, sizeof(int) * std::max(n, 1), stream));
+        h_ts.assign(n, TripState{0, 0});
+        h_status.assign(n, 0);
+        // capture one round trip of every resident solve
+        hipGraph_t gr = nullptr;
+        CK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        int rc = fp64 ? enqueue_trip<double>() : enqueue_trip<float>();
+        hipError_t e = hipStreamEndCapture(stream, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(OPO_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) return fail(OPO_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
+        dirty = false;
+        return OPO_OK;
+    }
+
+    template <typename T> int enqueue_trip() {
+        const SolveDesc<T> *sd = static_cast<const SolveDesc<T> *>(d_desc);
+        for (int k = 0; k < n_slices; ++k) {
+            for (const Range &r : trange)
+                CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off,
+                                        k == 0 ? BATCH_FIRST : BATCH_MID));
+            for (const Range &r : xyrange)
+                CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0));
+        }
+        for (const Range &r : trange)
+            CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST));
+        for (const Range &r : fsrange)
+            CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 1));
+        CK(launch_batch_power<T>(stream, prange.n, (int)resident.size(), sd, d_ts, d_status,
+                                 d_work + prange.off, d_partial, max_chunk));
+        return OPO_OK;
+    }
 };
 
 extern "C" {
@@ -763,22 +955,26 @@ size_t opo_batch_footprint(const opo_grid *g, int precision) {
     if (!g) return 0;
     const size_t n = (size_t)g->nx * g->ny * g->nt;
     const size_t cb = precision == OPO_FP64 ? 16 : 8;
-    const size_t tabs = (size_t)(3 * g->nx + 3 * g->ny + 5 * g->nx + 5 * g->nt + 2 * g->ny + 64) * cb;
+    // Api, Ap, As, Ai + tables (hi/lo pairs) + small per-solve buffers
+    const size_t tabs = 2 * (size_t)(6 * g->nx + 5 * g->ny + 5 * g->nt) * cb;
     return 4 * n * cb + tabs + 64 * 1024;
 }
 
 int opo_batch_create(int device, size_t budget, int precision, opo_batch **out) {
     if (!out) return fail(OPO_ERR_ARG, "null output");
     *out = nullptr;
+    if (precision != OPO_FP32 && precision != OPO_FP64) return fail(OPO_ERR_ARG, "bad precision");
     int ndev = opo_device_count();
     if (ndev < 1) return fail(OPO_ERR_NOGPU, "no HIP device");
     if (device < 0) device = 0;
     if (device >= ndev) return fail(OPO_ERR_ARG, "device out of range");
-    auto *b = new opo_batch;
+    std::unique_ptr<opo_batch> b(new opo_batch);
     b->device = device;
     b->budget = budget;
     b->prec = precision;
-    *out = b;
+    CK(hipSetDevice(device));
+    CK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    *out = b.release();
     return OPO_OK;
 }
 
@@ -791,9 +987,16 @@ int opo_batch_submit(opo_batch *b, const opo_grid *g, const opo_params *p, const
     int rc = validate(g, p);
     if (rc) return rc;
     if (p->precision != b->prec) return fail(OPO_ERR_ARG, "precision differs from the batch's");
+    if (!batch_supported(g->nx, g->ny, g->nt, p->precision == OPO_FP64))
+        return fail(OPO_ERR_UNSUPPORTED, "grid %dx%dx%d is not supported in a batch", g->nx, g->ny, g->nt);
+    if (p->dk_phase_scale != 0.0 && p->dk != 0.0)
+        return fail(OPO_ERR_UNSUPPORTED, "a batch runs solves without the dk phase term");
     const size_t fp = opo_batch_footprint(g, p->precision);
     if (fp > b->budget) return fail(OPO_ERR_BUDGET, "solve needs %zu bytes, budget is %zu", fp, b->budget);
     std::lock_guard<std::mutex> lk(b->mu);
+    if (b->n_slices >= 0 && p->n_slices != b->n_slices)
+        return fail(OPO_ERR_ARG, "all solves of a batch share n_slices (%d)", b->n_slices);
+    b->n_slices = p->n_slices;
     auto e = std::make_unique<BatchEntry>();
     e->ticket = b->next_ticket++;
     e->g = *g;
@@ -805,7 +1008,6 @@ int opo_batch_submit(opo_batch *b, const opo_grid *g, const opo_params *p, const
     e->api.assign((const unsigned char *)api, (const unsigned char *)api + fb);
     e->as.assign((const unsigned char *)as, (const unsigned char *)as + fb);
     e->ai.assign((const unsigned char *)ai, (const unsigned char *)ai + fb);
-    e->power.assign(3 * (size_t)n_trips, 0.0);
     if (ticket) *ticket = e->ticket;
     b->queue.push_back(e.get());
     b->all[e->ticket] = std::move(e);
@@ -815,49 +1017,62 @@ int opo_batch_submit(opo_batch *b, const opo_grid *g, const opo_params *p, const
 int opo_batch_step(opo_batch *b, int *n_active) {
     if (!b) return fail(OPO_ERR_ARG, "null batch");
     std::lock_guard<std::mutex> lk(b->mu);
-    // admit in arrival order while the head fits
+    CK(hipSetDevice(b->device));
+    if (n_active) *n_active = 0;
+    // admit in arrival order while the head of the queue fits the budget
     while (!b->queue.empty() && b->used + b->queue.front()->footprint <= b->budget) {
         BatchEntry *e = b->queue.front();
-        b->queue.pop_front();
         std::unique_ptr<SolveImpl> s;
         int rc = make_solve(&e->g, &e->p, b->device, s);
+        if (rc == OPO_ERR_NOMEM && !b->resident.empty()) break;  // wait for memory to free up
+        b->queue.pop_front();
         if (!rc) rc = s->set_fields(e->api.data(), e->as.data(), e->ai.data());
+        if (!rc && hipMalloc(&e->d_series, sizeof(double) * 3 * (size_t)e->n_trips) != hipSuccess)
+            rc = fail(OPO_ERR_NOMEM, "series allocation failed");
         if (rc) {
             e->finished = true;
             e->error = rc;
             e->error_msg = g_err;
             continue;
         }
-        e->api.clear(); e->api.shrink_to_fit();
-        e->as.clear(); e->as.shrink_to_fit();
-        e->ai.clear(); e->ai.shrink_to_fit();
+        e->api = {};
+        e->as = {};
+        e->ai = {};
         e->s = std::move(s);
         b->used += e->footprint;
         b->resident.push_back(e);
+        b->dirty = true;
     }
-    // one round trip of every resident solve, overlapped across streams
-    int stepped = 0;
-    for (BatchEntry *e : b->resident) {
-        int rc = e->s->enqueue_trips(1);
-        if (rc) { e->error = rc; e->error_msg = g_err; }
+    const int n = (int)b->resident.size();
+    if (n == 0) return OPO_OK;
+    int rc;
+    if (b->dirty && (rc = b->rebuild())) return rc;
+    for (int i = 0; i < n; ++i) {
+        const BatchEntry *e = b->resident[i];
+        const int rt = e->done_trips;
+        b->h_ts[i].trip = rt;
+        b->h_ts[i].fs = e->p.fs_on && !(e->p.skip_fs_on_last_trip && rt == e->p.n_round_trips - 1);
     }
+    CK(hipMemcpyAsync(b->d_ts, b->h_ts.data(), sizeof(TripState) * n, hipMemcpyHostToDevice, b->stream));
+    CK(hipGraphLaunch(b->graph, b->stream));
+    CK(hipMemcpyAsync(b->h_status.data(), b->d_status, sizeof(int) * n, hipMemcpyDeviceToHost, b->stream));
+    CK(hipStreamSynchronize(b->stream));
     std::vector<BatchEntry *> keep;
-    for (BatchEntry *e : b->resident) {
-        if (!e->error) {
-            int rc = e->s->finish_trips(1, &e->power[3 * (size_t)e->done_trips]);
-            if (rc == OPO_OK) e->done_trips++;
-            else { e->error = rc; e->error_msg = g_err; }
-            ++stepped;
-        }
-        if (e->error || e->done_trips >= e->n_trips) {
-            e->finished = true;
+    for (int i = 0; i < n; ++i) {
+        BatchEntry *e = b->resident[i];
+        if (b->h_status[i]) {
+            e->done_trips += 1;
+            e->error = OPO_ERR_NAN;
+            e->error_msg = "non-finite signal power at round trip " + std::to_string(e->done_trips - 1);
         } else {
-            keep.push_back(e);
+            e->done_trips += 1;
         }
+        if (e->error || e->done_trips >= e->n_trips) e->finished = true;
+        else keep.push_back(e);
     }
-    // finished solves keep their device memory until polled
+    if (keep.size() != b->resident.size()) b->dirty = true;
     b->resident.swap(keep);
-    if (n_active) *n_active = stepped;
+    if (n_active) *n_active = n;
     return OPO_OK;
 }
 
@@ -872,12 +1087,17 @@ int opo_batch_poll(opo_batch *b, uint64_t ticket, int *done, double *power, void
     if (!e->finished) return OPO_OK;
     int rc = e->error;
     std::string msg = e->error_msg;
-    if (power) std::memcpy(power, e->power.data(), sizeof(double) * 3 * (size_t)e->done_trips);
     if (e->s) {
+        CK(hipSetDevice(b->device));
+        if (power && e->done_trips > 0)
+            CK(hipMemcpy(power, e->d_series, sizeof(double) * 3 * (size_t)e->done_trips,
+                         hipMemcpyDeviceToHost));
         int r2 = e->s->get_fields(ap, as, ai);
         if (!rc) rc = r2;
         b->used -= e->footprint;
         e->s.reset();
+        (void)hipFree(e->d_series);
+        e->d_series = nullptr;
     }
     b->all.erase(it);
     if (rc) return fail(rc, "%s", msg.c_str());

@@ -16,6 +16,7 @@
 #include "opo_device.hpp"
 #include "opo_kernels.hpp"
 #include "opo_rk4.hpp"
+#include "opo_passes.hpp"
 
 namespace opo {
 
@@ -125,107 +126,12 @@ hipError_t launch_tline(hipStream_t st, const Geometry &g, const FieldOps<T> &op
     }
 }
 
-// ------------------------------------------------ fused nonlinear + t op
-// Blocks are handed to the 8 XCDs round-robin (block b runs on XCD b % 8).
-// Remap so that neighbouring tiles, which share cache lines at the tile
-// edges, are processed on the same XCD and hit its L2.
-__device__ __forceinline__ int xcd_local_tile(int b, int nb) {
-    if (nb % 8 != 0) return b;
-    return (b % 8) * (nb / 8) + b / 8;
-}
-
-// LDS budget of the fused pass: three tiles of NT x TX <= 48 KiB (two or
-// more workgroups per CU), at least 4 pixels wide.
-template <typename T, int NT> constexpr int tline3_tx() {
-    constexpr int per_px = 3 * NT * (int)sizeof(cplx<T>);
-    constexpr int tx = (per_px * 4 > 49152) ? 98304 / per_px : 49152 / per_px;
-    return tx >= 16 ? 16 : (tx >= 8 ? 8 : (tx >= 4 ? 4 : (tx >= 2 ? 2 : 1)));
-}
-
-template <typename T> struct alignas(4 * sizeof(T)) cpair {
-    cplx<T> a, b;
-};
-
 template <typename T, int NT, int TX, int BS, bool PHASE>
 __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
                                                const cplx<T> *__restrict__ tw, const int *status) {
     if (status && *status) return;
-    constexpr int TILE = NT * TX;
-    __shared__ cplx<T> lds[3 * TILE];
-    const int tid = threadIdx.x;
-    const int tiles_x = nx / TX;
-    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
-    const int y = tile / tiles_x, x0 = (tile % tiles_x) * TX;
-    const size_t plane = (size_t)nx * ny;
-    const size_t base = (size_t)y * nx + x0;
-    const Rk4Args<T> &rk = a.rk;
-    // pointwise step in registers, two pixels per thread per iteration
-    if constexpr (TX % 2 == 0) {
-        for (int i = 2 * tid; i < TILE; i += 2 * BS) {
-            const int t = i / TX, xl = i % TX;
-            const size_t gi = (size_t)t * plane + base + xl;
-            cpair<T> p = *reinterpret_cast<const cpair<T> *>(a.psrc + gi);
-            cpair<T> s = *reinterpret_cast<const cpair<T> *>(a.f[1] + gi);
-            cpair<T> q = *reinterpret_cast<const cpair<T> *>(a.f[2] + gi);
-            Tri<T> A0 = {p.a, s.a, q.a}, A1 = {p.b, s.b, q.b};
-            if (rk.nsteps > 0) {
-                rk4_half_step<T, PHASE>(rk, 0, A0);
-                rk4_half_step<T, PHASE>(rk, 0, A1);
-                if (rk.nsteps > 1) {
-                    rk4_half_step<T, PHASE>(rk, 1, A0);
-                    rk4_half_step<T, PHASE>(rk, 1, A1);
-                }
-            }
-            if (rk.out_scale) {
-                const cplx<T> ms = {rk.ms_re, rk.ms_im}, msl = {rk.ms_lo_re, rk.ms_lo_im};
-                const cplx<T> mi = {rk.mi_re, rk.mi_im}, mil = {rk.mi_lo_re, rk.mi_lo_im};
-                A0.s = cmul_hl(A0.s, ms, msl);
-                A1.s = cmul_hl(A1.s, ms, msl);
-                A0.i = cmul_hl(A0.i, mi, mil);
-                A1.i = cmul_hl(A1.i, mi, mil);
-            }
-            lds[i] = A0.p;
-            lds[i + 1] = A1.p;
-            lds[TILE + i] = A0.s;
-            lds[TILE + i + 1] = A1.s;
-            lds[2 * TILE + i] = A0.i;
-            lds[2 * TILE + i + 1] = A1.i;
-        }
-    } else {
-        for (int i = tid; i < TILE; i += BS) {
-            const int t = i / TX, xl = i % TX;
-            const size_t gi = (size_t)t * plane + base + xl;
-            Tri<T> A = {a.psrc[gi], a.f[1][gi], a.f[2][gi]};
-            if (rk.nsteps > 0) rk4_half_step<T, PHASE>(rk, 0, A);
-            if (rk.nsteps > 1) rk4_half_step<T, PHASE>(rk, 1, A);
-            if (rk.out_scale) {
-                A.s = cmul_hl(A.s, cplx<T>{rk.ms_re, rk.ms_im}, cplx<T>{rk.ms_lo_re, rk.ms_lo_im});
-                A.i = cmul_hl(A.i, cplx<T>{rk.mi_re, rk.mi_im}, cplx<T>{rk.mi_lo_re, rk.mi_lo_im});
-            }
-            lds[i] = A.p;
-            lds[TILE + i] = A.s;
-            lds[2 * TILE + i] = A.i;
-        }
-    }
-    __syncthreads();
-    const LineSet<T> ls{lds + a.first_field * TILE, TILE, a.tab0, a.tab_stride, 3 - a.first_field};
-    line_op_set<T, NT, +1, 1, TX>(ls, TX, tw, tid, BS);
-    if constexpr (TX % 2 == 0) {
-        for (int i = 2 * tid; i < TILE; i += 2 * BS) {
-            const int t = i / TX, xl = i % TX;
-            const size_t gi = (size_t)t * plane + base + xl;
-            *reinterpret_cast<cpair<T> *>(a.f[0] + gi) = cpair<T>{lds[i], lds[i + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[1] + gi) = cpair<T>{lds[TILE + i], lds[TILE + i + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[2] + gi) =
-                cpair<T>{lds[2 * TILE + i], lds[2 * TILE + i + 1]};
-        }
-    } else {
-        for (int i = tid; i < TILE; i += BS) {
-            const int t = i / TX, xl = i % TX;
-            const size_t gi = (size_t)t * plane + base + xl;
-            for (int f = 0; f < 3; ++f) a.f[f][gi] = lds[f * TILE + i];
-        }
-    }
+    __shared__ cplx<T> lds[3 * NT * TX];
+    nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds);
 }
 
 template <typename T, int NT>
@@ -268,18 +174,9 @@ template <typename T, int NX, int BS>
 __global__ void __launch_bounds__(BS) k_xyplane(FieldOps<T> ops, int nt, const cplx<T> *__restrict__ tw,
                                                 const int *status) {
     if (status && *status) return;
-    constexpr int NXP = NX + 1;  // padded row pitch
-    __shared__ cplx<T> lds[NX * NXP];
-    const int tid = threadIdx.x;
+    __shared__ cplx<T> lds[NX * (NX + 1)];
     const int t = blockIdx.x, f = blockIdx.y;
-    cplx<T> *A = ops.f[f] + (size_t)t * NX * NX;
-    for (int i = tid; i < NX * NX; i += BS) lds[(i / NX) * NXP + (i % NX)] = A[i];
-    __syncthreads();
-    // Q_x: lines are rows (pitch NXP), elements contiguous.
-    line_op<T, NX, -1, NXP, 1>(lds, NX, tw, ops.tab[f], tid, BS);
-    // Q_y: lines are columns, elements one row apart.
-    line_op<T, NX, -1, 1, NXP>(lds, NX, tw, ops.taby[f], tid, BS);
-    for (int i = tid; i < NX * NX; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
+    xy_plane<T, NX, BS>(ops.f[f] + (size_t)t * NX * NX, ops.tab[f], ops.taby[f], tw, lds);
 }
 
 template <typename T, int NX>

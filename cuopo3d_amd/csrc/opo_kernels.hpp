@@ -83,3 +83,61 @@ bool tline_supported(int nt, bool fp64);
 bool xyplane_supported(int nx, int ny, bool fp64);
 
 }  // namespace opo
+
+namespace opo {
+
+// ------------------------------------------------------------ batching
+// Everything a batched kernel needs about one solve (device memory).
+template <typename T> struct SolveDesc {
+    const cplx<T> *api;
+    cplx<T> *f[3];
+    const cplx<T> *twx, *twt;      // twiddles (hi/lo pairs)
+    const cplx<T> *tabT, *tabG;    // D tables (3 x 2nt, stride 2nt), GDD table
+    const cplx<T> *qx, *qy;        // Q factor tables (3 fields, stride 2nx / 2ny)
+    const cplx<T> *qfsx, *qfsy;    // free-space factor tables (s, i)
+    int nx, ny, nt, gdd_on;
+    T kp, ks, ki, c1, c3, h;       // nonlinear step
+    T ms_re, ms_im, ms_lo_re, ms_lo_im, mi_re, mi_im, mi_lo_re, mi_lo_im;  // mirror
+    double pref[3];                // power prefactors
+    double *series;                // per-trip power, 3 per trip
+    int series_cap;
+    int nchunk;                    // power partial sums per field
+};
+
+// Host-written per-trip state of a solve.
+struct TripState {
+    int trip;   // index of the trip being run (power goes to series[3*trip])
+    int fs;     // free-space step on this trip
+};
+
+// One workgroup's work: a solve and an item within it (tile, plane, chunk).
+struct WorkItem {
+    int solve;
+    int item;
+};
+
+enum { BATCH_FIRST = 0, BATCH_MID = 1, BATCH_LAST = 2 };
+
+// Fused pass for every work item of one NT class.  mode: BATCH_FIRST (pump
+// from Api, one half step), BATCH_MID (two half steps), BATCH_LAST (one half
+// step, mirror, GDD of s and i when the solve has it on).
+template <typename T>
+hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc<T> *sd,
+                             const int *status, const WorkItem *work, int mode);
+// xy operator for every work item of one NX class: item = f*nt + t over the
+// three fields (free_space = 0) or over s and i with the free-space tables
+// on solves whose TripState.fs is set (free_space = 1).
+template <typename T>
+hipError_t launch_batch_xy(hipStream_t st, int nx, int n_work, const SolveDesc<T> *sd,
+                           const TripState *ts, const int *status, const WorkItem *work,
+                           int free_space);
+// Per-solve power: partial sums (item = chunk) then one workgroup per solve
+// writing series[3*trip + f]; a non-finite signal power sets status.
+template <typename T>
+hipError_t launch_batch_power(hipStream_t st, int n_work, int n_solves, const SolveDesc<T> *sd,
+                              const TripState *ts, int *status, const WorkItem *work,
+                              double *partial, int max_chunk);
+int batch_tile_px(int nt, bool fp64);
+bool batch_supported(int nx, int ny, int nt, bool fp64);
+
+}  // namespace opo

@@ -1,0 +1,142 @@
+"""Continuous-batching request loop for OPO solves.
+
+Requests (grid, parameters, initial fields, number of round trips) arrive
+over time.  Each arrival is submitted to a resident opo_batch; every
+``step`` advances all admitted solves by one round trip, and solves that
+arrive while others run are admitted at the next round-trip boundary once
+the HBM budget has room (admission by footprint, in arrival order).
+Finished solves are collected as they complete.  Latency is measured from
+arrival to the moment the result (per-trip powers and fields) is on the
+host.
+
+The reference has no serving mode: its only multi-point driver is a bash
+loop starting one process per (power, waist) point (src/cuOPO3D.sh:56-86).
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Batch, configs
+from .capi import OPO_FP32
+
+
+@dataclass
+class Request:
+    arrival: float          # seconds after the start of the run
+    grid: object
+    params: object
+    fields: tuple
+    n_trips: int
+    tag: object = None
+    ticket: int = None
+    done_at: float = None
+    power: np.ndarray = None
+    out_fields: list = None
+    error: str = None
+
+    @property
+    def cell_steps(self):
+        g = self.grid
+        return g.nx * g.ny * g.nt * self.params.n_slices * self.n_trips
+
+
+@dataclass
+class ServeReport:
+    n: int
+    makespan_s: float
+    cell_steps: float
+    p50_s: float
+    p99_s: float
+    mean_s: float
+    steps: int
+    max_resident: int
+    latencies: list = field(default_factory=list)
+
+    @property
+    def rate(self):
+        return self.cell_steps / self.makespan_s if self.makespan_s > 0 else 0.0
+
+    def as_dict(self):
+        return {"requests": self.n, "makespan_s": self.makespan_s,
+                "cell_steps_per_s": self.rate, "p50_latency_s": self.p50_s,
+                "p99_latency_s": self.p99_s, "mean_latency_s": self.mean_s,
+                "batch_steps": self.steps, "max_resident": self.max_resident}
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return float("nan")
+    k = (len(xs) - 1) * q / 100.0
+    lo, hi = math.floor(k), math.ceil(k)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def serve(requests, budget_bytes, precision=OPO_FP32, keep_fields=True, clock=time.perf_counter):
+    """Run the requests through one continuously batched GPU queue."""
+    batch = Batch(budget_bytes, precision)
+    pending = sorted(requests, key=lambda r: r.arrival)
+    outstanding = {}
+    t0 = clock()
+    steps = 0
+    max_res = 0
+    i = 0
+    while i < len(pending) or outstanding:
+        now = clock() - t0
+        while i < len(pending) and pending[i].arrival <= now:
+            r = pending[i]
+            r.ticket = batch.submit(r.grid, r.params, *r.fields, r.n_trips)
+            outstanding[r.ticket] = r
+            i += 1
+        res, queued, _ = batch.stats()
+        if res == 0 and queued == 0:
+            if i < len(pending):   # idle until the next arrival
+                time.sleep(max(0.0, min(0.01, pending[i].arrival - (clock() - t0))))
+            continue
+        batch.step()
+        steps += 1
+        max_res = max(max_res, batch.stats()[0])
+        for t in list(outstanding):
+            r = outstanding[t]
+            try:
+                got = batch.poll(t, want_fields=keep_fields)
+            except RuntimeError as e:
+                r.error = str(e)
+                r.done_at = clock() - t0
+                batch.meta.pop(t, None)
+                del outstanding[t]
+                continue
+            if got is not None:
+                r.power, r.out_fields = got
+                r.done_at = clock() - t0
+                del outstanding[t]
+    makespan = clock() - t0
+    lat = [r.done_at - r.arrival for r in requests]
+    batch.close()
+    return ServeReport(n=len(requests), makespan_s=makespan,
+                       cell_steps=float(sum(r.cell_steps for r in requests if r.error is None)),
+                       p50_s=percentile(lat, 50), p99_s=percentile(lat, 99),
+                       mean_s=float(np.mean(lat)) if lat else float("nan"), steps=steps,
+                       max_resident=max_res, latencies=lat)
+
+
+def poisson_requests(n, rate_per_s, seed=5, grids=configs.SERVING_GRIDS, trips=(50, 300),
+                     precision=OPO_FP32):
+    """Requests with exponential inter-arrival gaps, grids uniform over
+    ``grids`` ((nxy, nt) pairs), trips uniform in ``trips``, physics drawn as
+    in the batched sweep."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    draws = configs.sweep_draws(n_solves=n, seed=seed + 1)
+    for k in range(n):
+        t += rng.exponential(1.0 / rate_per_s)
+        nxy, nt = grids[rng.integers(len(grids))]
+        n_trips = int(rng.integers(trips[0], trips[1] + 1))
+        d = dict(draws[k])
+        d["nxy"], d["nt"] = nxy, nt
+        g, p, ph, f = configs.sweep_solve(d, n_round_trips=n_trips, precision=precision)
+        out.append(Request(arrival=t, grid=g, params=p, fields=f, n_trips=n_trips, tag=k))
+    return out

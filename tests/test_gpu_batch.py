@@ -1,0 +1,162 @@
+"""Batched solves (opo_batch_*) on the GPU.
+
+Each solve's arithmetic in a batch is the same as alone, so a solve's fields
+must be bitwise equal whether it runs alone, in a mixed batch, or through
+the single-solve API.  Admission waits for HBM budget, an over-budget solve
+is rejected, and a solve that produces non-finite power is evicted without
+touching its neighbours.
+"""
+import numpy as np
+import pytest
+
+import cuopo3d_amd as co
+from cuopo3d_amd import configs
+from cuopo3d_amd.capi import OPO_FP32
+from oracle.pyoracle import Oracle
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GB = 1 << 30
+
+
+def mix(n=4, trips=3, seed=3):
+    out = []
+    draws = configs.sweep_draws(n_solves=16, seed=seed)
+    sizes = [(32, 128), (64, 64), (96, 128), (32, 256)]
+    for k in range(n):
+        d = dict(draws[k])
+        d["nxy"], d["nt"] = sizes[k % len(sizes)]
+        out.append(configs.sweep_solve(d, n_round_trips=trips))
+    return out
+
+
+def run_batch(solves, trips, budget=8 * GB):
+    b = co.Batch(budget)
+    tickets = [b.submit(g, p, *f, trips) for (g, p, ph, f) in solves]
+    steps = 0
+    while True:
+        res = [b.poll(t) if t in b.meta else None for t in tickets]
+        if all(t not in b.meta for t in tickets):
+            break
+        b.step()
+        steps += 1
+        assert steps < 10 * trips * len(solves) + 10
+    return b
+
+
+def collect(b, tickets):
+    return {t: b.poll(t) for t in tickets}
+
+
+def run_and_collect(solves, trips, budget=8 * GB):
+    b = co.Batch(budget)
+    tickets = [b.submit(g, p, *f, trips) for (g, p, ph, f) in solves]
+    results = {}
+    for _ in range(10 * trips * len(solves) + 10):
+        b.step()
+        for t in tickets:
+            if t not in results:
+                r = b.poll(t)
+                if r is not None:
+                    results[t] = r
+        if len(results) == len(tickets):
+            break
+    return [results[t] for t in tickets], b
+
+
+def test_batch_alone_equals_mixed_bitwise(require_gpu):
+    solves = mix(4, trips=3)
+    mixed, _ = run_and_collect(solves, 3)
+    for k, s in enumerate(solves):
+        alone, _ = run_and_collect([s], 3)
+        (pa, fa), (pm, fm) = alone[0], mixed[k]
+        for a, m in zip(fa, fm):
+            assert np.array_equal(a, m), k
+        assert np.array_equal(pa, pm), k
+    print("alone == mixed bitwise for", [(s[0].nx, s[0].nt) for s in solves], flush=True)
+
+
+def test_batch_equals_single_solve_and_oracle(require_gpu):
+    solves = mix(3, trips=2)
+    mixed, _ = run_and_collect(solves, 2)
+    for (g, p, ph, f), (pb, fb) in zip(solves, mixed):
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        ps = s.runOPO(2)
+        fs = s.get_fields()
+        for a, b in zip(fs, fb):
+            assert np.array_equal(a, b)
+        assert np.allclose(ps, pb, rtol=1e-13, atol=0)
+        o = Oracle(g, p)
+        o.set_fields(*f)
+        pr = o.run_round_trips(2)
+        fr = o.get_fields()
+        errs = [rel_l2(a, b) for a, b in zip(fb, fr)]
+        print((g.nx, g.nt), "batch vs oracle rel L2", errs, flush=True)
+        assert max(errs) < 1e-5
+        assert np.max(np.abs(pb - pr) / np.abs(pr)) < 1e-6
+
+
+def test_batch_admission_waits_for_budget(require_gpu):
+    solves = mix(3, trips=2)
+    fp = max(co.Batch(GB).footprint(g) for (g, _, _, _) in solves)
+    b = co.Batch(int(fp * 1.5))   # room for one solve at a time
+    tickets = [b.submit(g, p, *f, 2) for (g, p, ph, f) in solves]
+    b.step()
+    r, q, used = b.stats()
+    assert r == 1 and q == 2 and 0 < used <= fp * 1.5
+    results = {}
+    for _ in range(40):
+        b.step()
+        for t in tickets:
+            if t not in results:
+                x = b.poll(t)
+                if x is not None:
+                    results[t] = x
+        if len(results) == 3:
+            break
+    assert len(results) == 3
+    ref, _ = run_and_collect(solves, 2)
+    for t, (pr, fr) in zip(tickets, ref):
+        for a, c in zip(results[t][1], fr):
+            assert np.array_equal(a, c)
+
+
+def test_batch_rejects_over_budget(require_gpu):
+    (g, p, ph, f), = mix(1)
+    b = co.Batch(1 << 20)
+    with pytest.raises(RuntimeError, match="budget"):
+        b.submit(g, p, *f, 1)
+
+
+def test_batch_evicts_nan_solve_only(require_gpu):
+    solves = mix(3, trips=3)
+    g, p, ph, (api, as_, ai) = solves[1]
+    bad = as_.copy()
+    bad[0, 0, 0] = np.nan
+    poisoned = list(solves)
+    poisoned[1] = (g, p, ph, (api, bad, ai))
+    b = co.Batch(8 * GB)
+    tickets = [b.submit(gg, pp, *ff, 3) for (gg, pp, _, ff) in poisoned]
+    out = {}
+    for _ in range(20):
+        b.step()
+        for t in tickets:
+            if t in b.meta and t not in out:
+                try:
+                    r = b.poll(t)
+                except RuntimeError as e:
+                    out[t] = e
+                    b.meta.pop(t, None)
+                    continue
+                if r is not None:
+                    out[t] = r
+        if len(out) == 3:
+            break
+    assert isinstance(out[tickets[1]], RuntimeError) and "non-finite" in str(out[tickets[1]])
+    clean, _ = run_and_collect([solves[0], solves[2]], 3)
+    for t, ref in zip((tickets[0], tickets[2]), clean):
+        for a, c in zip(out[t][1], ref[1]):
+            assert np.array_equal(a, c)
