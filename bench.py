@@ -40,6 +40,13 @@ def main():
     ap.add_argument("--cpu-trips", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-trips", type=int, default=20)
+    ap.add_argument("--no-plain", action="store_true", help="skip the rocFFT comparator")
+    ap.add_argument("--sweep-trips", type=int, default=10, help="0 skips the batched sweep")
+    ap.add_argument("--serve-requests", type=int, default=16, help="0 skips the serving run")
+    ap.add_argument("--serve-rate", type=float, default=10.0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 128 MiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one solve per GPU on one GPU; --gpus {args.gpus} is not "
@@ -55,24 +62,26 @@ def main():
     if lib.opo_device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is only used for the device-wide barrier
-        sync = lambda: None  # noqa: E731
-
+    if args.calibrate_copy:
+        # a plain device-to-device copy of known size (Api -> Ap of a
+        # 128x128x1024 complex64 solve, 128 MiB): its FETCH_SIZE / WRITE_SIZE
+        # counters scale those of the split-step kernels to bytes
+        cg, cp, _, _ = configs.cavity_64(n=(128, 128, 1024), n_round_trips=1)
+        cs = co.Solver(cg, cp, device=0)
+        cs.set_fields(np.zeros((1024, 128, 128), np.complex64), None, None)
+        cs.sync()
+        cs.close()
     grid, p, ph, fields = configs.cavity_64(n_round_trips=args.warmup + args.steps)
     cells = grid.nx * grid.ny * grid.nt
     S = p.n_slices
     s = co.Solver(grid, p, device=0)
     s.set_fields(*fields)
+    tune_report = s.autotune(reps=20) if not args.no_tune else "not tuned"
     s.runOPO(args.warmup)
-    s.sync()
-    sync()
+    s.sync()   # all of the solve's work is on its stream
     t0 = time.perf_counter()
     pw = s.runOPO(args.steps)
     s.sync()
-    sync()
     t1 = time.perf_counter()
     dt = t1 - t0
     value = cells * S * args.steps / dt
@@ -108,6 +117,7 @@ def main():
         "stages": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in stages.items()},
         "gpu_kernel_ms_per_trip": gpu_kernel_ms,
+        "autotune": tune_report.strip().splitlines(),
         "signal_power_last_trip_W": float(pw[-1, 1]),
     }
 
@@ -134,7 +144,95 @@ def main():
                                          f"same 64x64x256 cavity workload ({c1 - c0:.1f} s)"}
         out["parity_vs_oracle"] = {"trips": args.cpu_trips, "field_rel_l2_p_s_i": errs,
                                    "signal_power_rel_max": perr}
+    if not args.no_plain:
+        out["plain_rocfft"] = plain_leg()
+        if "value" in out["plain_rocfft"]:
+            out["plain_rocfft"]["fused_speedup"] = value / out["plain_rocfft"]["value"]
+    if args.sweep_trips > 0:
+        out["sweep"] = sweep_leg(co, configs, args.sweep_trips)
+    if args.serve_requests > 0:
+        out["serving"] = serving_leg(args.serve_requests, args.serve_rate)
     print(json.dumps(out))
+
+
+def plain_leg(trips=4):
+    """rocFFT + separate elementwise kernels (torch) on the same cavity workload,
+    in a child process (torch carries its own HIP runtime)."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-m", "cuopo3d_amd.baseline_rocfft", "--trips", str(trips)],
+                       cwd=ROOT, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip().splitlines()[-1] if r.stderr.strip() else "failed"}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def sweep_leg(co, configs, trips, n_solves=64, plain_solves=8, plain_trips=1):
+    """64 uneven cavity solves batched (one launch per stage per size class, trip as a
+    graph) against the same 64 solves run one at a time through the single-solve
+    API, and against the plain rocFFT path one solve at a time (first solves)."""
+    draws = configs.sweep_draws(n_solves=n_solves, seed=3)
+    solves = [configs.sweep_solve(d, n_round_trips=trips) for d in draws]
+    co.autotune_grids([(g.nx, g.ny, g.nt) for (g, _, _, _) in solves])
+    work = sum(g.nx * g.ny * g.nt * p.n_slices for (g, p, _, _) in solves)
+    b = co.Batch(200 << 30)
+    tickets = [b.submit(g, p, *f, trips) for (g, p, _, f) in solves]
+    b.step()  # admission, graph capture, first trip (not timed)
+    t0 = time.perf_counter()
+    for _ in range(trips - 1):
+        b.step()
+    dt = time.perf_counter() - t0
+    batched = work * (trips - 1) / dt
+    for t in tickets:
+        b.poll(t, want_fields=False)
+    b.close()
+    # the same 64 solves one at a time, each warmed up by one trip
+    one_trips = 2
+    tot = 0.0
+    sub_rates = []
+    for k, (g, p, _, f) in enumerate(solves):
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        s.runOPO(1)
+        t0 = time.perf_counter()
+        s.runOPO(one_trips)
+        s.sync()
+        e = time.perf_counter() - t0
+        tot += e
+        if k < plain_solves:
+            sub_rates.append((g.nx * g.ny * g.nt * p.n_slices * one_trips, e))
+        s.close()
+    one = work * one_trips / tot
+    out = {"solves": n_solves, "trips_timed": trips - 1, "batched_cell_steps_per_s": batched,
+           "one_at_a_time_cell_steps_per_s": one, "speedup_vs_fused_one_at_a_time": batched / one}
+    # plain rocFFT path, one solve at a time, in a child process
+    import subprocess
+    r = subprocess.run([sys.executable, "-m", "cuopo3d_amd.baseline_rocfft", "--sweep-solves",
+                        str(plain_solves), "--trips", str(plain_trips)],
+                       cwd=ROOT, capture_output=True, text=True, timeout=900)
+    if r.returncode == 0:
+        pl = json.loads(r.stdout.strip().splitlines()[-1])
+        fused_sub = sum(w for w, _ in sub_rates) / sum(e for _, e in sub_rates)
+        out["plain_rocfft_one_at_a_time_cell_steps_per_s"] = pl["value"]
+        out["plain_rocfft_sample"] = f"first {plain_solves} solves of the sweep, {plain_trips} trip each"
+        # the plain rate on the sample, scaled to the whole set by the fused one-at-a-time
+        # rates of the same sample and of the whole set (the sizes differ)
+        out["plain_rocfft_whole_set_estimate"] = pl["value"] * one / fused_sub
+        out["speedup_vs_plain_one_at_a_time"] = batched / out["plain_rocfft_whole_set_estimate"]
+    else:
+        out["plain_rocfft_error"] = r.stderr.strip().splitlines()[-1] if r.stderr.strip() else "failed"
+    return out
+
+
+def serving_leg(n, rate):
+    import cuopo3d_amd as co
+    from cuopo3d_amd import serving, configs
+    co.autotune_grids([(nxy, nxy, nt) for nxy, nt in configs.SERVING_GRIDS])
+    reqs = serving.poisson_requests(n, rate_per_s=rate, seed=5)
+    rep = serving.serve(reqs, budget_bytes=128 << 30, keep_fields=False)
+    d = rep.as_dict()
+    d["arrival_rate_per_s"] = rate
+    d["errors"] = sum(1 for r in reqs if r.error)
+    return d
 
 
 if __name__ == "__main__":

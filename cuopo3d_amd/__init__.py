@@ -14,7 +14,7 @@ from . import capi
 from .capi import (OPO_FP32, OPO_FP64, OPO_OK, OPO_ERR_NAN, OPO_ERR_UNSUPPORTED,  # noqa: F401
                    Grid, Params, Physical, Crystal, check, load)
 
-__all__ = ["Solver", "Batch", "Grid", "Params", "Physical", "load", "check",
+__all__ = ["Solver", "Batch", "autotune_grids", "Grid", "Params", "Physical", "load", "check",
            "OPO_FP32", "OPO_FP64"]
 
 
@@ -127,6 +127,37 @@ class Solver:
 
     def sync(self):
         check(self.lib.opo_sync(self.h), "opo_sync")
+
+    def autotune(self, reps=10):
+        """Pick the fastest launch variants for this grid (opo_autotune)."""
+        buf = C.create_string_buffer(4096)
+        check(self.lib.opo_autotune(self.h, reps, buf, len(buf)), "opo_autotune")
+        return buf.value.decode()
+
+
+def autotune_grids(grids, precision=OPO_FP32, reps=10, device=0):
+    """Tune the launch variants for a set of (NX, NY, NT) grids, e.g. the
+    size classes of a sweep or a serving mix, before they are run.
+
+    The xy pass is tuned per NX; the t pass per NT on the widest grid of
+    that NT (it carries most of the work), last, so that later calls do not
+    overwrite it.  Returns the tuner's reports."""
+    from . import configs
+    grids = sorted({(int(g[0]), int(g[1]), int(g[2])) for g in grids})
+    widest = {}
+    for nx, ny, nt in grids:
+        widest[nt] = max(widest.get(nt, (0, 0, 0)), (nx, ny, nt))
+    by_nx = {}
+    for g in grids:
+        by_nx.setdefault(g[0], g)
+    order = [g for g in by_nx.values() if g not in widest.values()] + list(widest.values())
+    reports = []
+    for nx, ny, nt in order:
+        grid, p, _, fields = configs.cavity_64(n=(nx, ny, nt), n_round_trips=1, precision=precision)
+        with Solver(grid, p, device=device) as s:
+            s.set_fields(*fields)
+            reports.append(s.autotune(reps))
+    return reports
 
 
 class Batch:

@@ -22,6 +22,8 @@ OPO_ERR_NOGPU = -8
 
 OPO_FP32 = 0
 OPO_FP64 = 1
+OPO_TUNE_TIME_PASS = 0
+OPO_TUNE_XY_PASS = 1
 
 STAGES = ("nonlinear", "time_op", "xy_op", "free_space", "gdd", "power")
 
@@ -137,6 +139,9 @@ def load():
         "opo_courant_ok": (ip, [C.POINTER(Params)]),
         "opo_sync": (ip, [vp]),
         "opo_profile_trips": (ip, [vp, ip, dp, C.POINTER(ip)]),
+        "opo_autotune": (ip, [vp, ip, C.c_char_p, sz]),
+        "opo_tune_get": (ip, [ip, ip, ip]),
+        "opo_tune_set": (ip, [ip, ip, ip, ip]),
         "opo_batch_create": (ip, [ip, sz, ip, C.POINTER(vp)]),
         "opo_batch_destroy": (None, [vp]),
         "opo_batch_submit": (ip, [vp, C.POINTER(Grid), C.POINTER(Params), vp, vp, vp, ip,

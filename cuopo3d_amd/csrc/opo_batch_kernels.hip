@@ -16,12 +16,11 @@ namespace opo {
 template <typename T>
 __device__ __forceinline__ Rk4Args<T> batch_rk(const SolveDesc<T> &d, int mode) {
     Rk4Args<T> r{};
-    r.kp = d.kp;
-    r.ks = d.ks;
-    r.ki = d.ki;
-    r.c1 = d.c1;
-    r.c3 = d.c3;
-    r.h = d.h;
+    for (int f = 0; f < 3; ++f) {
+        r.kc1[f] = d.kc1[f];
+        r.kc3[f] = d.kc3[f];
+        r.kh6[f] = d.kh6[f];
+    }
     r.nsteps = mode == BATCH_MID ? 2 : 1;
     r.has_phase = 0;
     r.out_scale = mode == BATCH_LAST ? 1 : 0;
@@ -139,14 +138,7 @@ __global__ void __launch_bounds__(64) kb_power_final(const SolveDesc<T> *__restr
     if (!isfinite(pw[1])) status[s] = k + 1;
 }
 
-int batch_tile_px(int nt, bool fp64) {
-    switch (nt) {
-#define TXC(N) case N: return fp64 ? tline3_tx<double, N>() : tline3_tx<float, N>();
-        TXC(16) TXC(32) TXC(64) TXC(128) TXC(256) TXC(512) TXC(1024)
-#undef TXC
-        default: return 0;
-    }
-}
+int batch_tile_px(int nt, bool fp64) { return tune_tile_px(fp64, nt, tune_get(fp64, 0, nt)); }
 
 bool batch_supported(int nx, int ny, int nt, bool fp64) {
     if (nx != ny) return false;
@@ -162,10 +154,20 @@ bool batch_supported(int nx, int ny, int nt, bool fp64) {
 template <typename T, int NT>
 static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const int *status,
                               const WorkItem *work, int mode) {
-    constexpr int TX = tline3_tx<T, NT>();
-    hipLaunchKernelGGL((kb_time<T, NT, TX, 256>), dim3((unsigned)n_work), dim3(256), 0, st, sd, status,
-                       work, mode);
-    return hipGetLastError();
+    const int v = tune_get(sizeof(T) == 8, 0, NT);
+    hipError_t e = hipErrorInvalidValue;
+    static_for<0, kTimeVariants>([&](auto vv) {
+        constexpr int V = decltype(vv)::value;
+        using Vr = TimeVariant<T, NT, V>;
+        if constexpr (Vr::valid) {
+            if (V == v) {
+                hipLaunchKernelGGL((kb_time<T, NT, Vr::tx, Vr::bs>), dim3((unsigned)n_work), dim3(Vr::bs), 0,
+                                   st, sd, status, work, mode);
+                e = hipGetLastError();
+            }
+        }
+    });
+    return e;
 }
 
 template <typename T>
@@ -187,8 +189,16 @@ hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc
 template <typename T, int NX>
 static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
                               const int *status, const WorkItem *work, int fs) {
-    constexpr int BS = NX >= 96 ? 512 : 256;
-    hipLaunchKernelGGL((kb_xy<T, NX, BS>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts, status, work, fs);
+    const dim3 g((unsigned)n_work);
+    if constexpr (sizeof(T) == 8) {
+        constexpr int BS = NX >= 96 ? 512 : 256;
+        hipLaunchKernelGGL((kb_xy<T, NX, BS>), g, dim3(BS), 0, st, sd, ts, status, work, fs);
+    } else {
+        const int v = tune_get(false, 1, NX);
+        if (v == 0) hipLaunchKernelGGL((kb_xy<T, NX, 256>), g, dim3(256), 0, st, sd, ts, status, work, fs);
+        else if (v == 1) hipLaunchKernelGGL((kb_xy<T, NX, 512>), g, dim3(512), 0, st, sd, ts, status, work, fs);
+        else hipLaunchKernelGGL((kb_xy<T, NX, 1024>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
+    }
     return hipGetLastError();
 }
 

@@ -125,6 +125,7 @@ struct SolveImpl {
     virtual int profile_trips(int n, double *ms, int *launches) = 0;
     // Write this solve's SolveDesc<T> (T = the solve's precision) to out.
     virtual void fill_desc(void *out, double *series, int series_cap, int nchunk) const = 0;
+    virtual int autotune(int reps, std::string &report) = 0;
     size_t field_bytes() const { return N * (prec == OPO_FP64 ? 16 : 8); }
     int select() {
         CK(hipSetDevice(device));
@@ -290,12 +291,12 @@ template <typename T> struct SolveT final : SolveImpl {
     Rk4Args<T> rk_args(int nsteps, double z0, double z1, bool mirror) const {
         Rk4Args<T> a{};
         const double h = 0.5 * p.dz;
-        a.kp = (T)p.kappa[0];
-        a.ks = (T)p.kappa[1];
-        a.ki = (T)p.kappa[2];
-        a.c1 = (T)(p.rk4_h_in_stages ? 0.5 * h : 0.5);
-        a.c3 = (T)(p.rk4_h_in_stages ? h : 1.0);
-        a.h = (T)h;
+        const double c1 = p.rk4_h_in_stages ? 0.5 * h : 0.5, c3 = p.rk4_h_in_stages ? h : 1.0;
+        for (int f = 0; f < 3; ++f) {
+            a.kc1[f] = (T)(p.kappa[f] * c1);
+            a.kc3[f] = (T)(p.kappa[f] * c3);
+            a.kh6[f] = (T)(p.kappa[f] * h / 6.0);
+        }
         a.nsteps = nsteps;
         a.has_phase = (p.dk_phase_scale != 0.0 && p.dk != 0.0) ? 1 : 0;
         for (int st = 0; st < 2; ++st) {
@@ -390,6 +391,12 @@ template <typename T> struct SolveT final : SolveImpl {
         a.tab_stride = tab_stride;
         a.first_field = first;
         a.rk = rk;
+        if (g.nx > 1 && !rk.has_phase) {
+            const int tx = tune_tile_px(sizeof(T) == 8, g.nt, tune_get(sizeof(T) == 8, 0, g.nt));
+            if (tx == 0 || g.nx % tx != 0)
+                return fail(OPO_ERR_UNSUPPORTED, "tuned tile of %d pixels does not divide NX=%d (NT=%d)", tx,
+                            g.nx, g.nt);
+        }
         PROF(stage, (launch_tline3<T>(stream, geo(), a, tw_t, st)));
         return OPO_OK;
     }
@@ -503,7 +510,7 @@ template <typename T> struct SolveT final : SolveImpl {
             case STEP_MIRROR: {
                 // mirror alone: a zero-coupling RK4 pass is the identity, then the factors.
                 Rk4Args<T> a = rk_args(1, 0.0, 0.0, true);
-                a.kp = a.ks = a.ki = (T)0;
+                for (int f = 0; f < 3; ++f) a.kc1[f] = a.kc3[f] = a.kh6[f] = (T)0;
                 rc = q_rk4(A[0], a, nullptr);
                 break;
             }
@@ -590,6 +597,75 @@ template <typename T> struct SolveT final : SolveImpl {
         return OPO_OK;
     }
 
+    // Time every variant of the two passes on this grid, keep the fastest.
+    int autotune(int reps, std::string &report) override {
+        int rc = select();
+        if (rc) return rc;
+        if (reps < 1) reps = 1;
+        const bool fp64 = sizeof(T) == 8;
+        const size_t fb = N * sizeof(C);
+        C *save[3] = {nullptr, nullptr, nullptr};
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&]() {
+            for (auto *b : save) (void)hipFree(b);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        };
+        for (int f = 0; f < 3; ++f) {
+            if (hipMalloc(&save[f], fb) != hipSuccess) {
+                cleanup();
+                return fail(OPO_ERR_NOMEM, "autotune: no room to save the fields");
+            }
+            CK(hipMemcpyAsync(save[f], A[f], fb, hipMemcpyDeviceToDevice, stream));
+        }
+        CK(hipEventCreate(&e0));
+        CK(hipEventCreate(&e1));
+        const Rk4Args<T> rk = rk_args(2, 0.0, p.dz, false);
+        char buf[160];
+        for (int axis = 0; axis < 2; ++axis) {
+            const int n = axis == 0 ? g.nt : g.nx;
+            int best = tune_get(fp64, axis, n);
+            double best_t = 1e30;
+            report += axis == 0 ? "t pass NT=" : "xy pass NX=";
+            report += std::to_string(n) + ":";
+            for (int v = 0; v < tune_variant_count(axis); ++v) {
+                if (!tune_variant_valid(fp64, axis, n, v)) continue;
+                if (axis == 0 && (g.nx == 1 || g.nx % tune_tile_px(fp64, n, v) != 0)) continue;
+                tune_set(fp64, axis, n, v);
+                auto launch = [&]() {
+                    return axis == 0 ? q_nl_time(A[0], rk, tabT[0], (int)(tabT[1] - tabT[0]), 0,
+                                                 OPO_STAGE_TIME_OP, nullptr)
+                                     : q_diffraction(nullptr);
+                };
+                if ((rc = launch())) { cleanup(); return rc; }
+                CK(hipEventRecord(e0, stream));
+                for (int r = 0; r < reps && !rc; ++r) rc = launch();
+                if (rc) { cleanup(); return rc; }
+                CK(hipEventRecord(e1, stream));
+                CK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                CK(hipEventElapsedTime(&ms, e0, e1));
+                const double us = 1e3 * ms / reps;
+                std::snprintf(buf, sizeof(buf), " v%d(tile %d, block %d) %.2f us;", v,
+                              axis == 0 ? tune_tile_px(fp64, n, v) : n, tune_block(fp64, axis, n, v), us);
+                report += buf;
+                if (us < best_t) {
+                    best_t = us;
+                    best = v;
+                }
+            }
+            tune_set(fp64, axis, n, best);
+            report += " -> v" + std::to_string(best) + "\n";
+        }
+        for (int f = 0; f < 3; ++f)
+            CK(hipMemcpyAsync(A[f], save[f], fb, hipMemcpyDeviceToDevice, stream));
+        CK(hipStreamSynchronize(stream));
+        cleanup();
+        for (auto &gx : graph)   // re-capture with the chosen variants
+            if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+        return OPO_OK;
+    }
+
     void fill_desc(void *out, double *series_out, int cap, int nchunk) const override {
         SolveDesc<T> d{};
         d.api = Api;
@@ -607,12 +683,11 @@ template <typename T> struct SolveT final : SolveImpl {
         d.nt = g.nt;
         d.gdd_on = p.gdd_on;
         const Rk4Args<T> r = rk_args(1, 0.0, 0.0, true);
-        d.kp = r.kp;
-        d.ks = r.ks;
-        d.ki = r.ki;
-        d.c1 = r.c1;
-        d.c3 = r.c3;
-        d.h = r.h;
+        for (int f = 0; f < 3; ++f) {
+            d.kc1[f] = r.kc1[f];
+            d.kc3[f] = r.kc3[f];
+            d.kh6[f] = r.kh6[f];
+        }
         d.ms_re = r.ms_re;
         d.ms_im = r.ms_im;
         d.ms_lo_re = r.ms_lo_re;
@@ -736,6 +811,15 @@ struct opo_batch {
     int n_slices = -1, max_chunk = 1;
     hipGraphExec_t graph = nullptr;
     bool dirty = true;
+    std::vector<int> built_variants;  // tuned launch variants the graph uses
+
+    std::vector<int> current_variants() const {
+        std::vector<int> v;
+        const bool fp64 = prec == OPO_FP64;
+        for (const Range &r : trange) v.push_back(tune_get(fp64, 0, r.cls));
+        for (const Range &r : xyrange) v.push_back(tune_get(fp64, 1, r.cls));
+        return v;
+    }
     std::vector<TripState> h_ts;
     std::vector<int> h_status;
 
@@ -800,6 +884,9 @@ struct opo_batch {
             const int tx = batch_tile_px(kv.first, fp64);
             for (int i : kv.second) {
                 const opo_grid &g = resident[i]->g;
+                if (tx == 0 || g.nx % tx != 0)
+                    return fail(OPO_ERR_UNSUPPORTED, "tuned tile of %d pixels does not divide NX=%d (NT=%d)",
+                                tx, g.nx, kv.first);
                 const int tiles = (g.nx / tx) * g.ny;
                 for (int t = 0; t < tiles; ++t) work.push_back(WorkItem{i, t});
             }
@@ -847,6 +934,7 @@ struct opo_batch {
         e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
         if (e != hipSuccess) return fail(OPO_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
+        built_variants = current_variants();
         dirty = false;
         return OPO_OK;
     }
@@ -941,6 +1029,23 @@ int opo_profile_trips(opo_solve *s, int n, double ms[OPO_N_STAGES], int launches
     SOLVE_OR_FAIL(s);
     if (n < 1 || !ms || !launches) return fail(OPO_ERR_ARG, "opo_profile_trips: bad argument");
     return s->s->profile_trips(n, ms, launches);
+}
+int opo_autotune(opo_solve *s, int reps, char *report, size_t report_len) {
+    SOLVE_OR_FAIL(s);
+    std::string rep;
+    int rc = s->s->autotune(reps, rep);
+    if (report && report_len) {
+        std::snprintf(report, report_len, "%s", rep.c_str());
+    }
+    return rc;
+}
+int opo_tune_get(int precision, int axis, int n) {
+    return tune_get(precision == OPO_FP64, axis, n);
+}
+int opo_tune_set(int precision, int axis, int n, int variant) {
+    if (!tune_set(precision == OPO_FP64, axis, n, variant))
+        return fail(OPO_ERR_ARG, "no variant %d for axis %d, size %d", variant, axis, n);
+    return OPO_OK;
 }
 int opo_sync(opo_solve *s) {
     SOLVE_OR_FAIL(s);
@@ -1046,6 +1151,7 @@ int opo_batch_step(opo_batch *b, int *n_active) {
     const int n = (int)b->resident.size();
     if (n == 0) return OPO_OK;
     int rc;
+    if (!b->dirty && b->current_variants() != b->built_variants) b->dirty = true;  // re-tuned
     if (b->dirty && (rc = b->rebuild())) return rc;
     for (int i = 0; i < n; ++i) {
         const BatchEntry *e = b->resident[i];

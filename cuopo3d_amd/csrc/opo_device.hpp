@@ -138,7 +138,7 @@ constexpr double sin2pi(long long j, long long n) {
 
 template <int V> using ic = std::integral_constant<int, V>;
 
-template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f) {
+template <int B, int E, typename F> __host__ __device__ __forceinline__ void static_for(F &&f) {
     if constexpr (B < E) {
         f(ic<B>{});
         static_for<B + 1, E>(f);

@@ -134,23 +134,143 @@ __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
     nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds);
 }
 
+// ----------------------------------------------------- variant tables
+static int g_time_var[2][11];   // [fp64][log2 NT]
+static int g_xy_var[2][8];      // [fp64][NX index]
+static bool g_tune_init = false;
+
+static int log2i(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return (1 << l) == n ? l : -1;
+}
+static int nx_index(int nx) {
+    switch (nx) {
+        case 1: return 0; case 16: return 1; case 32: return 2; case 64: return 3;
+        case 96: return 4; case 128: return 5; default: return -1;
+    }
+}
+static void tune_init() {
+    if (g_tune_init) return;
+    for (int f = 0; f < 2; ++f) {
+        for (int k = 0; k < 11; ++k) g_time_var[f][k] = 0;
+        const int nxs[6] = {1, 16, 32, 64, 96, 128};
+        for (int k = 0; k < 6; ++k) g_xy_var[f][k] = xy_default_variant(nxs[k]);
+    }
+    g_tune_init = true;
+}
+
+template <typename T, int NT>
+static void time_variant_info(int v, bool &valid, int &tx, int &bs) {
+    valid = false;
+    static_for<0, kTimeVariants>([&](auto vv) {
+        constexpr int V = decltype(vv)::value;
+        using Vr = TimeVariant<T, NT, V>;
+        if (V == v) {
+            valid = Vr::valid;
+            tx = Vr::tx;
+            bs = Vr::bs;
+        }
+    });
+}
+
+template <typename T>
+static bool time_info(int nt, int v, int &tx, int &bs) {
+    bool valid = false;
+    switch (nt) {
+#define TI(N) case N: time_variant_info<T, N>(v, valid, tx, bs); break;
+        TI(1) TI(16) TI(32) TI(64) TI(128) TI(256) TI(512) TI(1024)
+#undef TI
+        default: return false;
+    }
+    return valid;
+}
+
+int tune_variant_count(int axis) { return axis == 0 ? kTimeVariants : kXyVariants; }
+
+bool tune_variant_valid(bool fp64, int axis, int n, int v) {
+    if (axis == 0) {
+        if (v < 0 || v >= kTimeVariants) return false;
+        int tx = 0, bs = 0;
+        return fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
+    }
+    if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
+    if (fp64) return v == xy_default_variant(n);
+    return n >= 16 || v == 0;
+}
+
+int tune_get(bool fp64, int axis, int n) {
+    tune_init();
+    if (axis == 0) {
+        const int l = log2i(n);
+        return (l >= 0 && l < 11) ? g_time_var[fp64][l] : 0;
+    }
+    const int k = nx_index(n);
+    return k >= 0 ? g_xy_var[fp64][k] : 0;
+}
+
+bool tune_set(bool fp64, int axis, int n, int v) {
+    tune_init();
+    if (!tune_variant_valid(fp64, axis, n, v)) return false;
+    if (axis == 0) g_time_var[fp64][log2i(n)] = v;
+    else g_xy_var[fp64][nx_index(n)] = v;
+    return true;
+}
+
+int tune_tile_px(bool fp64, int nt, int v) {
+    int tx = 0, bs = 0;
+    const bool ok = fp64 ? time_info<double>(nt, v, tx, bs) : time_info<float>(nt, v, tx, bs);
+    return ok ? tx : 0;
+}
+
+int tune_block(bool fp64, int axis, int n, int v) {
+    if (axis == 1) return xy_variant_bs(v);
+    int tx = 0, bs = 0;
+    const bool ok = fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
+    return ok ? bs : 0;
+}
+
+template <typename T, int NT, int V>
+static hipError_t tline3_launch_v(hipStream_t st, const Geometry &g, const TLine3Args<T> &a,
+                                  const cplx<T> *tw, const int *status) {
+    using Vr = TimeVariant<T, NT, V>;
+    if constexpr (!Vr::valid) {
+        return hipErrorInvalidValue;
+    } else {
+        if (g.nx % Vr::tx != 0) return hipErrorInvalidValue;
+        dim3 grid((unsigned)((g.nx / Vr::tx) * g.ny));
+        if (a.rk.has_phase) {
+            if constexpr (V == 0)
+                hipLaunchKernelGGL((k_tline3<T, NT, Vr::tx, Vr::bs, true>), grid, dim3(Vr::bs), 0, st, a,
+                                   g.nx, g.ny, tw, status);
+            else
+                return hipErrorInvalidValue;
+        } else {
+            hipLaunchKernelGGL((k_tline3<T, NT, Vr::tx, Vr::bs, false>), grid, dim3(Vr::bs), 0, st, a, g.nx,
+                               g.ny, tw, status);
+        }
+        return hipGetLastError();
+    }
+}
+
 template <typename T, int NT>
 static hipError_t tline3_dispatch(hipStream_t st, const Geometry &g, const TLine3Args<T> &a,
                                   const cplx<T> *tw, const int *status) {
-    constexpr int TX = tline3_tx<T, NT>();
-    const bool ph = a.rk.has_phase != 0;
-    if (g.nx % TX == 0) {
-        dim3 grid((unsigned)((g.nx / TX) * g.ny));
-        if (ph) hipLaunchKernelGGL((k_tline3<T, NT, TX, 256, true>), grid, dim3(256), 0, st, a, g.nx, g.ny, tw, status);
-        else hipLaunchKernelGGL((k_tline3<T, NT, TX, 256, false>), grid, dim3(256), 0, st, a, g.nx, g.ny, tw, status);
-    } else if (g.nx == 1) {
+    if (g.nx == 1) {
         dim3 grid((unsigned)g.ny);
-        if (ph) hipLaunchKernelGGL((k_tline3<T, NT, 1, 256, true>), grid, dim3(256), 0, st, a, g.nx, g.ny, tw, status);
-        else hipLaunchKernelGGL((k_tline3<T, NT, 1, 256, false>), grid, dim3(256), 0, st, a, g.nx, g.ny, tw, status);
-    } else {
-        return hipErrorInvalidValue;
+        if (a.rk.has_phase)
+            hipLaunchKernelGGL((k_tline3<T, NT, 1, 256, true>), grid, dim3(256), 0, st, a, g.nx, g.ny, tw, status);
+        else
+            hipLaunchKernelGGL((k_tline3<T, NT, 1, 256, false>), grid, dim3(256), 0, st, a, g.nx, g.ny, tw, status);
+        return hipGetLastError();
     }
-    return hipGetLastError();
+    const int v = a.rk.has_phase ? 0 : tune_get(sizeof(T) == 8, 0, NT);
+    hipError_t e = hipErrorInvalidValue;
+    static_for<0, kTimeVariants>([&](auto vv) {
+        constexpr int V = decltype(vv)::value;
+        if (V == v) e = tline3_launch_v<T, NT, V>(st, g, a, tw, status);
+    });
+    return e;
 }
 
 template <typename T>
@@ -183,8 +303,15 @@ template <typename T, int NX>
 static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                               const cplx<T> *tw, const int *status) {
     dim3 grid((unsigned)g.nt, (unsigned)nf);
-    constexpr int BS = NX >= 96 ? 512 : 256;
-    hipLaunchKernelGGL((k_xyplane<T, NX, BS>), grid, dim3(BS), 0, st, ops, g.nt, tw, status);
+    const int v = tune_get(sizeof(T) == 8, 1, NX);
+    if constexpr (NX < 16 || sizeof(T) == 8) {
+        constexpr int BS = NX >= 96 ? 512 : 256;
+        hipLaunchKernelGGL((k_xyplane<T, NX, BS>), grid, dim3(BS), 0, st, ops, g.nt, tw, status);
+    } else {
+        if (v == 0) hipLaunchKernelGGL((k_xyplane<T, NX, 256>), grid, dim3(256), 0, st, ops, g.nt, tw, status);
+        else if (v == 1) hipLaunchKernelGGL((k_xyplane<T, NX, 512>), grid, dim3(512), 0, st, ops, g.nt, tw, status);
+        else hipLaunchKernelGGL((k_xyplane<T, NX, 1024>), grid, dim3(1024), 0, st, ops, g.nt, tw, status);
+    }
     return hipGetLastError();
 }
 

@@ -11,9 +11,10 @@ template <typename T> struct cplx;
 // Arguments of the pointwise nonlinear step (reference solverRK4,
 // Solver3D.cuh:872-943; right-hand side dAdz, :365-382).
 template <typename T> struct Rk4Args {
-    T kp, ks, ki;      // kappa of pump, signal, idler (1/V)
-    T c1, c3;          // stage coefficients: A + c1 k1, A + c1 k2, A + c3 k3
-    T h;               // half-step dz/2
+    // Per field (p, s, i), with kappa the coupling and h = dz/2: kappa*c1 and
+    // kappa*c3 (stage points A + c1 k1, A + c1 k2, A + c3 k3) and kappa*h/6
+    // (the update weight), formed in FP64.
+    T kc1[3], kc3[3], kh6[3];
     int nsteps;        // consecutive half steps fused (1 or 2)
     int has_phase;     // dk phase present (dk_phase_scale != 0)
     T es_re[2][3], es_im[2][3];  // e^{+i s dk z} at the 3 stage positions of each step
@@ -79,6 +80,15 @@ hipError_t launch_power_final(hipStream_t st, const double *partial, int nblk, d
                               double pref1, double pref2, double *series, int *trip, int *status,
                               int check_nan, int series_cap);
 
+// Launch-variant tables (the autotuner's choices), per precision and size.
+// axis 0: fused t pass, keyed by NT; axis 1: xy pass, keyed by NX.
+int tune_variant_count(int axis);
+bool tune_variant_valid(bool fp64, int axis, int n, int v);
+int tune_get(bool fp64, int axis, int n);
+bool tune_set(bool fp64, int axis, int n, int v);
+int tune_tile_px(bool fp64, int nt, int v);   // pixels per tile of a t-pass variant
+int tune_block(bool fp64, int axis, int n, int v);
+
 bool tline_supported(int nt, bool fp64);
 bool xyplane_supported(int nx, int ny, bool fp64);
 
@@ -96,7 +106,7 @@ template <typename T> struct SolveDesc {
     const cplx<T> *qx, *qy;        // Q factor tables (3 fields, stride 2nx / 2ny)
     const cplx<T> *qfsx, *qfsy;    // free-space factor tables (s, i)
     int nx, ny, nt, gdd_on;
-    T kp, ks, ki, c1, c3, h;       // nonlinear step
+    T kc1[3], kc3[3], kh6[3];      // nonlinear step weights (see Rk4Args)
     T ms_re, ms_im, ms_lo_re, ms_lo_im, mi_re, mi_im, mi_lo_re, mi_lo_im;  // mirror
     double pref[3];                // power prefactors
     double *series;                // per-trip power, 3 per trip

@@ -28,6 +28,26 @@ template <typename T, int NT> constexpr int tline3_tx() {
     return tx >= 16 ? 16 : (tx >= 8 ? 8 : (tx >= 4 ? 4 : (tx >= 2 ? 2 : 1)));
 }
 
+// Launch variants of the fused pass for one NT (chosen by the autotuner):
+// V = 2 m + b with tile width m = 0: TXd, 1: TXd/2, 2: 2 TXd pixels and
+// b = 0: 256, 1: 512 threads.  Only variant 0 exists for FP64 and for
+// NT < 64; a variant whose three tiles exceed 96 KiB of LDS does not exist.
+constexpr int kTimeVariants = 6;
+template <typename T, int NT, int V> struct TimeVariant {
+    static constexpr int txd = tline3_tx<T, NT>();
+    static constexpr int m = V / 2, b = V % 2;
+    static constexpr int tx = m == 0 ? txd : (m == 1 ? txd / 2 : txd * 2);
+    static constexpr int bs = b == 0 ? 256 : 512;
+    static constexpr bool valid =
+        V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 &&
+                   3 * NT * tx * (int)sizeof(cplx<T>) <= 98304);
+};
+
+// Block-size variants of the xy pass: 256, 512 or 1024 threads.
+constexpr int kXyVariants = 3;
+constexpr int xy_variant_bs(int v) { return v == 0 ? 256 : (v == 1 ? 512 : 1024); }
+constexpr int xy_default_variant(int nx) { return nx >= 96 ? 1 : 0; }
+
 template <typename T> struct alignas(4 * sizeof(T)) cpair {
     cplx<T> a, b;
 };

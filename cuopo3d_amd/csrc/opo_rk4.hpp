@@ -14,48 +14,57 @@ template <typename T> struct Tri {
     cplx<T> p, s, i;
 };
 
-// dAdz (Solver3D.cuh:365-382): i kp As Ai e^{-i s dk z}, i ks Ap Ai* e^{+i s dk z},
-// i ki Ap As* e^{+i s dk z}; products evaluated in the reference's order.
+// dAdz (Solver3D.cuh:365-382): dAp = i kp As Ai e^{-i s dk z},
+// dAs = i ks Ap Ai* e^{+i s dk z}, dAi = i ki Ap As* e^{+i s dk z}.
+// The slopes are carried without their factor i*kappa (q = k / (i kappa));
+// kappa is folded into the real stage and update weights, so a stage point
+// A + c k is A + i (c kappa) q and costs two FMAs per field.
 template <typename T, bool PHASE>
-__device__ __forceinline__ Tri<T> rhs(const Rk4Args<T> &a, int st, int stage, const Tri<T> &A) {
-    Tri<T> d;
-    cplx<T> ip = {-a.kp * A.s.y, a.kp * A.s.x};  // (i kp) As
-    cplx<T> is = {-a.ks * A.p.y, a.ks * A.p.x};  // (i ks) Ap
-    cplx<T> ii = {-a.ki * A.p.y, a.ki * A.p.x};  // (i ki) Ap
-    d.p = cmul(ip, A.i);
-    d.s = cmulc(is, A.i);
-    d.i = cmulc(ii, A.s);
+__device__ __forceinline__ Tri<T> slopes(const Rk4Args<T> &a, int st, int stage, const Tri<T> &A) {
+    Tri<T> q;
+    q.p = cmul(A.s, A.i);
+    q.s = cmulc(A.p, A.i);
+    q.i = cmulc(A.p, A.s);
     if constexpr (PHASE) {
         cplx<T> e = {a.es_re[st][stage], a.es_im[st][stage]};
-        d.p = cmulc(d.p, e);
-        d.s = cmul(d.s, e);
-        d.i = cmul(d.i, e);
+        q.p = cmulc(q.p, e);
+        q.s = cmul(q.s, e);
+        q.i = cmul(q.i, e);
     }
-    return d;
+    return q;
 }
 
-template <typename T> __device__ __forceinline__ Tri<T> axpy(const Tri<T> &A, const Tri<T> &k, T c) {
-    // kernelLinealCombination (Solver3D.cuh:386-402): A + k*s
-    return {cadd(A.p, cscale(k.p, c)), cadd(A.s, cscale(k.s, c)), cadd(A.i, cscale(k.i, c))};
+// a + i w q for a real weight w
+template <typename T> __device__ __forceinline__ cplx<T> add_iw(cplx<T> a, cplx<T> q, T w) {
+    return {a.x - w * q.y, a.y + w * q.x};
 }
 
-template <typename T> __device__ __forceinline__ cplx<T> rk4_comb(cplx<T> a, cplx<T> k1, cplx<T> k2,
-                                                                  cplx<T> k3, cplx<T> k4, T h) {
-    // rk4 (Solver3D.cuh:418-420): A += (k1 + 2 k2 + 2 k3 + k4) * dz / 6
-    const T two = (T)2, six = (T)6;
-    cplx<T> s = cadd(cadd(cadd(k1, cscale(k2, two)), cscale(k3, two)), k4);
-    return {a.x + (s.x * h) / six, a.y + (s.y * h) / six};
+// kernelLinealCombination (Solver3D.cuh:386-402): the stage point A + c k
+template <typename T>
+__device__ __forceinline__ Tri<T> stage_point(const Tri<T> &A, const Tri<T> &q, const T *w) {
+    return {add_iw(A.p, q.p, w[0]), add_iw(A.s, q.s, w[1]), add_iw(A.i, q.i, w[2])};
+}
+
+// rk4 (Solver3D.cuh:418-420): A += (k1 + 2 k2 + 2 k3 + k4) h / 6, here
+// A + i (kappa h/6) (q1 + 2 q2 + 2 q3 + q4) with kappa h/6 formed in FP64 on
+// the host (no divisions in the kernel).
+template <typename T>
+__device__ __forceinline__ cplx<T> rk4_comb(cplx<T> a, cplx<T> q1, cplx<T> q2, cplx<T> q3, cplx<T> q4,
+                                            T w) {
+    const T two = (T)2;
+    cplx<T> s = cadd(cadd(cadd(q1, cscale(q2, two)), cscale(q3, two)), q4);
+    return add_iw(a, s, w);
 }
 
 template <typename T, bool PHASE>
 __device__ __forceinline__ void rk4_half_step(const Rk4Args<T> &a, int st, Tri<T> &A) {
-    Tri<T> k1 = rhs<T, PHASE>(a, st, 0, A);
-    Tri<T> k2 = rhs<T, PHASE>(a, st, 1, axpy(A, k1, a.c1));
-    Tri<T> k3 = rhs<T, PHASE>(a, st, 1, axpy(A, k2, a.c1));
-    Tri<T> k4 = rhs<T, PHASE>(a, st, 2, axpy(A, k3, a.c3));
-    A.p = rk4_comb(A.p, k1.p, k2.p, k3.p, k4.p, a.h);
-    A.s = rk4_comb(A.s, k1.s, k2.s, k3.s, k4.s, a.h);
-    A.i = rk4_comb(A.i, k1.i, k2.i, k3.i, k4.i, a.h);
+    Tri<T> q1 = slopes<T, PHASE>(a, st, 0, A);
+    Tri<T> q2 = slopes<T, PHASE>(a, st, 1, stage_point(A, q1, a.kc1));
+    Tri<T> q3 = slopes<T, PHASE>(a, st, 1, stage_point(A, q2, a.kc1));
+    Tri<T> q4 = slopes<T, PHASE>(a, st, 2, stage_point(A, q3, a.kc3));
+    A.p = rk4_comb(A.p, q1.p, q2.p, q3.p, q4.p, a.kh6[0]);
+    A.s = rk4_comb(A.s, q1.s, q2.s, q3.s, q4.s, a.kh6[1]);
+    A.i = rk4_comb(A.i, q1.i, q2.i, q3.i, q4.i, a.kh6[2]);
 }
 
 }  // namespace opo

@@ -194,6 +194,22 @@ int opo_sync(opo_solve *s);
 int opo_profile_trips(opo_solve *s, int n, double ms[OPO_N_STAGES],
                       int launches[OPO_N_STAGES]);
 
+/* Launch-variant autotuner (no counterpart in the reference, whose launch
+ * shapes are compile-time constants, DtypesConsts.cuh:45-48).  For the
+ * solve's grid and precision, times every compiled variant of the fused
+ * t pass (tile width x block size) and of the xy pass (block size), `reps`
+ * launches each, and keeps the fastest for all later launches of that size
+ * in this process.  The solve's fields are saved and restored.  Results do
+ * not depend on the variant (tiles only group lines).  report (nullable)
+ * receives a one-line summary per pass. */
+#define OPO_TUNE_TIME_PASS 0
+#define OPO_TUNE_XY_PASS   1
+int opo_autotune(opo_solve *s, int reps, char *report, size_t report_len);
+/* Current variant for (precision, axis, size n = NT or NX); set returns
+ * OPO_ERR_ARG for a variant that does not exist. */
+int opo_tune_get(int precision, int axis, int n);
+int opo_tune_set(int precision, int axis, int n, int variant);
+
 /* ------------------------------------------------- batching and serving */
 
 /* A batch holds many solves of different grids resident in HBM and advances

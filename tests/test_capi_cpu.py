@@ -69,3 +69,15 @@ def test_batch_footprint():
     g = Grid(128, 128, 1024)
     fp = lib.opo_batch_footprint(C.byref(g), 0)
     assert 4 * 8 * 128 * 128 * 1024 <= fp < 4 * 8 * 128 * 128 * 1024 * 1.01
+
+
+def test_tune_set_rejects_missing_variants():
+    """The variant tables are host state: checkable without a GPU."""
+    from cuopo3d_amd.capi import OPO_FP32, OPO_TUNE_TIME_PASS, OPO_TUNE_XY_PASS
+    lib = capi.load()
+    keep = lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, 256)
+    assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, 256, 99) != capi.OPO_OK
+    assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, 16, 1) != capi.OPO_OK  # NT < 64: one variant
+    assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 48, 0) != capi.OPO_OK    # no such NX
+    assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, 256, keep) == capi.OPO_OK
+    assert lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, 256) == keep

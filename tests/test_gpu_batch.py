@@ -32,24 +32,6 @@ def mix(n=4, trips=3, seed=3):
     return out
 
 
-def run_batch(solves, trips, budget=8 * GB):
-    b = co.Batch(budget)
-    tickets = [b.submit(g, p, *f, trips) for (g, p, ph, f) in solves]
-    steps = 0
-    while True:
-        res = [b.poll(t) if t in b.meta else None for t in tickets]
-        if all(t not in b.meta for t in tickets):
-            break
-        b.step()
-        steps += 1
-        assert steps < 10 * trips * len(solves) + 10
-    return b
-
-
-def collect(b, tickets):
-    return {t: b.poll(t) for t in tickets}
-
-
 def run_and_collect(solves, trips, budget=8 * GB):
     b = co.Batch(budget)
     tickets = [b.submit(g, p, *f, trips) for (g, p, ph, f) in solves]
@@ -100,7 +82,10 @@ def test_batch_equals_single_solve_and_oracle(require_gpu):
 
 
 def test_batch_admission_waits_for_budget(require_gpu):
-    solves = mix(3, trips=2)
+    draws = configs.sweep_draws(n_solves=3, seed=9)
+    for d in draws:
+        d["nxy"], d["nt"] = 64, 128      # equal footprints
+    solves = [configs.sweep_solve(d, n_round_trips=2) for d in draws]
     fp = max(co.Batch(GB).footprint(g) for (g, _, _, _) in solves)
     b = co.Batch(int(fp * 1.5))   # room for one solve at a time
     tickets = [b.submit(g, p, *f, 2) for (g, p, ph, f) in solves]

@@ -145,3 +145,36 @@ def test_femto_single_pass(require_gpu):
     pg, pr = s.averagePowerCutXY(), o.power()
     print("power rel err", np.abs(pg - pr) / pr, flush=True)
     assert np.all(np.abs(pg - pr) / pr < 1e-6)
+
+
+def test_precision_study_femto(require_gpu):
+    """FP32 bulk vs the all-FP64 device path vs the FP64 oracle on the
+    femtosecond single pass (64x64x1024, 399 slices)."""
+    grid, p32, ph, f32 = configs.femto_64(precision=OPO_FP32)
+    _, p64, _, f64 = configs.femto_64(precision=OPO_FP64)
+    # the same initial fields for every leg: the complex64 values, widened
+    f64 = tuple(x.astype(np.complex128) for x in f32)
+    o = Oracle(grid, p64)
+    o.set_fields(*f64)
+    o.ssfm()
+    r = o.get_fields()
+    s32 = co.Solver(grid, p32, device=0)
+    s32.set_fields(*f32)
+    s32.runSinglePass()
+    g32 = s32.get_fields()
+    s64 = co.Solver(grid, p64, device=0)
+    s64.set_fields(*f64)
+    s64.runSinglePass()
+    g64 = s64.get_fields()
+    e32 = [rel_l2(a, b) for a, b in zip(g32, r)]
+    e64 = [rel_l2(a, b) for a, b in zip(g64, r)]
+    e3264 = [rel_l2(a, b) for a, b in zip(g32, g64)]
+    pw = o.power()
+    pe32 = np.abs(s32.averagePowerCutXY() - pw) / pw
+    pe64 = np.abs(s64.averagePowerCutXY() - pw) / pw
+    print("precision study femto 64x64x1024, 399 slices", flush=True)
+    print("  fp32 vs oracle  field", e32, "power", pe32, flush=True)
+    print("  fp64 vs oracle  field", e64, "power", pe64, flush=True)
+    print("  fp32 vs fp64    field", e3264, flush=True)
+    assert max(e32) < 1e-5 and max(pe32) < 1e-6
+    assert max(e64) < 1e-11 and max(pe64) < 1e-12

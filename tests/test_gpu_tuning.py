@@ -1,0 +1,97 @@
+"""Launch variants and the autotuner (opo_autotune, opo_tune_get/set).
+
+A variant changes only how lines are grouped into workgroups (tile width,
+block size), never the arithmetic of a line, so every variant must give the
+same fields; the tuner must leave the solve's fields as it found them.
+"""
+import numpy as np
+import pytest
+
+import cuopo3d_amd as co
+from cuopo3d_amd import configs
+from cuopo3d_amd.capi import OPO_FP32, OPO_OK, OPO_TUNE_TIME_PASS, OPO_TUNE_XY_PASS
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = [(64, 64, 256), (32, 32, 1024), (128, 128, 64), (16, 16, 512)]
+
+
+@pytest.fixture
+def restore_tuning():
+    lib = co.load()
+    saved = {}
+    for nt in (16, 32, 64, 128, 256, 512, 1024):
+        saved[(OPO_TUNE_TIME_PASS, nt)] = lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, nt)
+    for nx in (16, 32, 64, 96, 128):
+        saved[(OPO_TUNE_XY_PASS, nx)] = lib.opo_tune_get(OPO_FP32, OPO_TUNE_XY_PASS, nx)
+    yield lib
+    for (axis, n), v in saved.items():
+        assert lib.opo_tune_set(OPO_FP32, axis, n, v) == OPO_OK
+
+
+def _valid(lib, axis, n):
+    """Variants that exist for size n."""
+    keep = lib.opo_tune_get(OPO_FP32, axis, n)
+    out = []
+    for v in range(8):
+        if lib.opo_tune_set(OPO_FP32, axis, n, v) == OPO_OK:
+            out.append(v)
+    lib.opo_tune_set(OPO_FP32, axis, n, keep)
+    return out
+
+
+def _run(grid, p, fields, trips=2):
+    with co.Solver(grid, p, device=0) as s:
+        s.set_fields(*fields)
+        pw = s.runOPO(trips)
+        return s.get_fields(), pw
+
+
+@pytest.mark.parametrize("n", GRIDS, ids=lambda n: "x".join(map(str, n)))
+def test_variants_give_the_same_fields(n, restore_tuning, require_gpu):
+    lib = restore_tuning
+    nx, ny, nt = n
+    grid, p, _, fields = configs.cavity_64(n=n, n_round_trips=2)
+    lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, nt, 0)
+    lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, nx, 0)
+    ref, pref = _run(grid, p, fields)
+    for axis, size in ((OPO_TUNE_TIME_PASS, nt), (OPO_TUNE_XY_PASS, nx)):
+        for v in _valid(lib, axis, size):
+            assert lib.opo_tune_set(OPO_FP32, axis, size, v) == OPO_OK
+            try:
+                got, pw = _run(grid, p, fields)
+            except RuntimeError as e:  # a tile that does not divide NX is refused, not run
+                assert "does not divide" in str(e), e
+                continue
+            errs = [rel_l2(a, b) for a, b in zip(got, ref)]
+            same = all(np.array_equal(a, b) for a, b in zip(got, ref))
+            print(f"{n} axis {axis} v{v}: bitwise {same}, rel L2 {max(errs):.1e}", flush=True)
+            assert max(errs) < 1e-6
+            assert np.max(np.abs(pw - pref) / pref) < 1e-6
+        lib.opo_tune_set(OPO_FP32, axis, size, 0)
+
+
+def test_autotune_keeps_fields_and_picks_a_variant(restore_tuning, require_gpu):
+    lib = restore_tuning
+    grid, p, _, fields = configs.cavity_64(n=(64, 64, 256), n_round_trips=3)
+    ref, pref = _run(grid, p, fields, trips=3)
+    with co.Solver(grid, p, device=0) as s:
+        s.set_fields(*fields)
+        s.runOPO(1)
+        before = s.get_fields()
+        report = s.autotune(reps=3)
+        print(report, flush=True)
+        assert "t pass NT=256" in report and "xy pass NX=64" in report
+        after = s.get_fields()
+        for a, b in zip(before, after):
+            assert np.array_equal(a, b)
+        # the run continues with the tuned variants and matches the untuned run
+        s.runOPO(2)
+        got = s.get_fields()
+    for a, b in zip(got, ref):
+        assert rel_l2(a, b) < 1e-6
+    v = lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, 256)
+    assert v in _valid(lib, OPO_TUNE_TIME_PASS, 256)
+
