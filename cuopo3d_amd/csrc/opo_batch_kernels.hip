@@ -61,6 +61,91 @@ __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ s
     nl_time_tile<T, NT, TX, BS, false>(a, d.nx, d.ny, w.item, d.twt, lds);
 }
 
+// Work units of the fused pass over a work list: tile w.item of solve w.solve.
+template <typename T, int NT, int TX> struct TimeBatch {
+    const SolveDesc<T> *sd;
+    const int *status;
+    const WorkItem *work;
+    int n, mode;
+    __device__ int units() const { return n; }
+    __device__ bool skip(int u) const { return status[work[u].solve] != 0; }
+    __device__ int key(int u) const { return work[u].solve; }
+    __device__ int nx(int u) const { return sd[work[u].solve].nx; }
+    __device__ int ny(int u) const { return sd[work[u].solve].ny; }
+    __device__ int tile(int u) const { return work[u].item; }
+    __device__ TLine3Args<T> args(int u) const {
+        const SolveDesc<T> &d = sd[work[u].solve];
+        TLine3Args<T> a;
+        a.psrc = mode == BATCH_FIRST ? d.api : d.f[0];
+        a.f[0] = d.f[0];
+        a.f[1] = d.f[1];
+        a.f[2] = d.f[2];
+        if (mode == BATCH_LAST) {
+            a.tab0 = d.tabG;
+            a.tab_stride = 0;
+            a.first_field = d.gdd_on ? 1 : 3;
+        } else {
+            a.tab0 = d.tabT;
+            a.tab_stride = 2 * NT;
+            a.first_field = 0;
+        }
+        a.rk = batch_rk(d, mode);
+        return a;
+    }
+    __device__ void stage(int u, cplx<T> *stab, int tid, int nthr) const {
+        stage_time_tables<T, NT>(sd[work[u].solve].twt, args(u), stab, tid, nthr);
+    }
+};
+
+template <typename T, int NT, int TX, int BS>
+__global__ void __launch_bounds__(BS) kb_timep(const SolveDesc<T> *__restrict__ sd,
+                                               const int *__restrict__ status,
+                                               const WorkItem *__restrict__ work, int n_work, int mode) {
+    __shared__ cplx<T> lds[3 * NT * TX];
+    __shared__ cplx<T> stab[8 * NT];
+    nl_time_run<T, NT, TX, BS>(TimeBatch<T, NT, TX>{sd, status, work, n_work, mode}, lds, stab);
+}
+
+// Work units of the xy pass over a work list: item = k nt + t, plane t of
+// field k (or of s and i with the free-space tables).
+template <typename T, int NX> struct XyBatch {
+    const SolveDesc<T> *sd;
+    const TripState *ts;
+    const int *status;
+    const WorkItem *work;
+    int n, free_space;
+    __device__ int units() const { return n; }
+    __device__ bool skip(int u) const {
+        const int s = work[u].solve;
+        return status[s] != 0 || (free_space && !ts[s].fs);
+    }
+    __device__ int key(int u) const { return work[u].solve; }
+    __device__ int slot(int u) const { return work[u].item / sd[work[u].solve].nt; }
+    __device__ cplx<T> *plane(int u) const {
+        const SolveDesc<T> &d = sd[work[u].solve];
+        const int k = work[u].item / d.nt, t = work[u].item % d.nt;
+        const int f = free_space ? 1 + k : k;
+        return (f == 0 ? d.f[0] : (f == 1 ? d.f[1] : d.f[2])) + (size_t)t * NX * NX;
+    }
+    __device__ void stage(int u, cplx<T> *stab, int tid, int nthr) const {
+        const SolveDesc<T> &d = sd[work[u].solve];
+        const cplx<T> *bx = free_space ? d.qfsx : d.qx, *by = free_space ? d.qfsy : d.qy;
+        const cplx<T> *tx[3] = {bx, bx + 2 * NX, bx + 4 * NX};
+        const cplx<T> *ty[3] = {by, by + 2 * NX, by + 4 * NX};
+        stage_xy_tables<T, NX>(d.twx, tx, ty, free_space ? 2 : 3, stab, tid, nthr);
+    }
+};
+
+template <typename T, int NX, int BS>
+__global__ void __launch_bounds__(BS) kb_xyp(const SolveDesc<T> *__restrict__ sd,
+                                             const TripState *__restrict__ ts,
+                                             const int *__restrict__ status,
+                                             const WorkItem *__restrict__ work, int n_work, int free_space) {
+    __shared__ cplx<T> lds[NX * (NX + 1)];
+    __shared__ cplx<T> stab[14 * NX];
+    xy_run<T, NX, BS>(XyBatch<T, NX>{sd, ts, status, work, n_work, free_space}, lds, stab);
+}
+
 template <typename T, int NX, int BS>
 __global__ void __launch_bounds__(BS) kb_xy(const SolveDesc<T> *__restrict__ sd,
                                             const TripState *__restrict__ ts,
@@ -161,8 +246,12 @@ static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
         using Vr = TimeVariant<T, NT, V>;
         if constexpr (Vr::valid) {
             if (V == v) {
-                hipLaunchKernelGGL((kb_time<T, NT, Vr::tx, Vr::bs>), dim3((unsigned)n_work), dim3(Vr::bs), 0,
-                                   st, sd, status, work, mode);
+                if constexpr (Vr::persistent)
+                    hipLaunchKernelGGL((kb_timep<T, NT, Vr::tx, Vr::bs>), dim3(persistent_grid(n_work, Vr::per_cu)),
+                                       dim3(Vr::bs), 0, st, sd, status, work, n_work, mode);
+                else
+                    hipLaunchKernelGGL((kb_time<T, NT, Vr::tx, Vr::bs>), dim3((unsigned)n_work), dim3(Vr::bs), 0,
+                                       st, sd, status, work, mode);
                 e = hipGetLastError();
             }
         }
@@ -195,9 +284,19 @@ static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
         hipLaunchKernelGGL((kb_xy<T, NX, BS>), g, dim3(BS), 0, st, sd, ts, status, work, fs);
     } else {
         const int v = tune_get(false, 1, NX);
-        if (v == 0) hipLaunchKernelGGL((kb_xy<T, NX, 256>), g, dim3(256), 0, st, sd, ts, status, work, fs);
-        else if (v == 1) hipLaunchKernelGGL((kb_xy<T, NX, 512>), g, dim3(512), 0, st, sd, ts, status, work, fs);
-        else hipLaunchKernelGGL((kb_xy<T, NX, 1024>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
+        const int bs = xy_variant_bs(v);
+        if (xy_variant_persistent(v)) {
+            const dim3 pg(persistent_grid(n_work, xy_variant_per_cu(v)));
+            if (bs == 256) hipLaunchKernelGGL((kb_xyp<T, NX, 256>), pg, dim3(256), 0, st, sd, ts, status, work, n_work, fs);
+            else if (bs == 512) hipLaunchKernelGGL((kb_xyp<T, NX, 512>), pg, dim3(512), 0, st, sd, ts, status, work, n_work, fs);
+            else hipLaunchKernelGGL((kb_xyp<T, NX, 1024>), pg, dim3(1024), 0, st, sd, ts, status, work, n_work, fs);
+        } else if (bs == 256) {
+            hipLaunchKernelGGL((kb_xy<T, NX, 256>), g, dim3(256), 0, st, sd, ts, status, work, fs);
+        } else if (bs == 512) {
+            hipLaunchKernelGGL((kb_xy<T, NX, 512>), g, dim3(512), 0, st, sd, ts, status, work, fs);
+        } else {
+            hipLaunchKernelGGL((kb_xy<T, NX, 1024>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
+        }
     }
     return hipGetLastError();
 }

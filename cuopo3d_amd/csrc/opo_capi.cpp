@@ -646,8 +646,11 @@ template <typename T> struct SolveT final : SolveImpl {
                 float ms = 0.f;
                 CK(hipEventElapsedTime(&ms, e0, e1));
                 const double us = 1e3 * ms / reps;
-                std::snprintf(buf, sizeof(buf), " v%d(tile %d, block %d) %.2f us;", v,
-                              axis == 0 ? tune_tile_px(fp64, n, v) : n, tune_block(fp64, axis, n, v), us);
+                const int pc = tune_per_cu(fp64, axis, n, v);
+                char pers[32] = "";
+                if (pc) std::snprintf(pers, sizeof(pers), ", persistent %d/CU", pc);
+                std::snprintf(buf, sizeof(buf), " v%d(tile %d, block %d%s) %.2f us;", v,
+                              axis == 0 ? tune_tile_px(fp64, n, v) : n, tune_block(fp64, axis, n, v), pers, us);
                 report += buf;
                 if (us < best_t) {
                     best_t = us;

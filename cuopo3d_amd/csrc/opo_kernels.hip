@@ -134,6 +134,34 @@ __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
     nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds);
 }
 
+template <typename T, int NT, int TX, int BS>
+__global__ void __launch_bounds__(BS) k_tline3p(TLine3Args<T> a, int nx, int ny,
+                                                const cplx<T> *__restrict__ tw, const int *status) {
+    if (status && *status) return;
+    __shared__ cplx<T> lds[3 * NT * TX];
+    __shared__ cplx<T> stab[8 * NT];
+    nl_time_run<T, NT, TX, BS>(TimeSolo<T, NT, TX>{a, nx, ny, tw}, lds, stab);
+}
+
+// Compute units of the current device (persistent grids are sized by it).
+static int cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
+unsigned persistent_grid(int units, int per_cu) {
+    const long long g = (long long)per_cu * cu_count();
+    return (unsigned)(units < g ? units : g);
+}
+
 // ----------------------------------------------------- variant tables
 static int g_time_var[2][11];   // [fp64][log2 NT]
 static int g_xy_var[2][8];      // [fp64][NX index]
@@ -161,7 +189,7 @@ static void tune_init() {
 }
 
 template <typename T, int NT>
-static void time_variant_info(int v, bool &valid, int &tx, int &bs) {
+static void time_variant_info(int v, bool &valid, int &tx, int &bs, int &per_cu) {
     valid = false;
     static_for<0, kTimeVariants>([&](auto vv) {
         constexpr int V = decltype(vv)::value;
@@ -170,20 +198,27 @@ static void time_variant_info(int v, bool &valid, int &tx, int &bs) {
             valid = Vr::valid;
             tx = Vr::tx;
             bs = Vr::bs;
+            per_cu = Vr::per_cu;
         }
     });
 }
 
 template <typename T>
-static bool time_info(int nt, int v, int &tx, int &bs) {
+static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu) {
     bool valid = false;
     switch (nt) {
-#define TI(N) case N: time_variant_info<T, N>(v, valid, tx, bs); break;
+#define TI(N) case N: time_variant_info<T, N>(v, valid, tx, bs, per_cu); break;
         TI(1) TI(16) TI(32) TI(64) TI(128) TI(256) TI(512) TI(1024)
 #undef TI
         default: return false;
     }
     return valid;
+}
+
+template <typename T>
+static bool time_info(int nt, int v, int &tx, int &bs) {
+    int per_cu = 0;
+    return time_info<T>(nt, v, tx, bs, per_cu);
 }
 
 int tune_variant_count(int axis) { return axis == 0 ? kTimeVariants : kXyVariants; }
@@ -196,7 +231,9 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
     }
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
     if (fp64) return v == xy_default_variant(n);
-    return n >= 16 || v == 0;
+    if (n < 16) return v == 0;
+    if (xy_variant_persistent(v)) return n * n / xy_variant_bs(v) <= 16;
+    return true;
 }
 
 int tune_get(bool fp64, int axis, int n) {
@@ -223,6 +260,13 @@ int tune_tile_px(bool fp64, int nt, int v) {
     return ok ? tx : 0;
 }
 
+int tune_per_cu(bool fp64, int axis, int n, int v) {
+    if (axis == 1) return xy_variant_per_cu(v);
+    int tx = 0, bs = 0, per_cu = 0;
+    const bool ok = fp64 ? time_info<double>(n, v, tx, bs, per_cu) : time_info<float>(n, v, tx, bs, per_cu);
+    return ok ? per_cu : 0;
+}
+
 int tune_block(bool fp64, int axis, int n, int v) {
     if (axis == 1) return xy_variant_bs(v);
     int tx = 0, bs = 0;
@@ -238,7 +282,14 @@ static hipError_t tline3_launch_v(hipStream_t st, const Geometry &g, const TLine
         return hipErrorInvalidValue;
     } else {
         if (g.nx % Vr::tx != 0) return hipErrorInvalidValue;
-        dim3 grid((unsigned)((g.nx / Vr::tx) * g.ny));
+        const int units = (g.nx / Vr::tx) * g.ny;
+        if constexpr (Vr::persistent) {
+            if (a.rk.has_phase) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_tline3p<T, NT, Vr::tx, Vr::bs>), dim3(persistent_grid(units, Vr::per_cu)),
+                               dim3(Vr::bs), 0, st, a, g.nx, g.ny, tw, status);
+            return hipGetLastError();
+        }
+        dim3 grid((unsigned)units);
         if (a.rk.has_phase) {
             if constexpr (V == 0)
                 hipLaunchKernelGGL((k_tline3<T, NT, Vr::tx, Vr::bs, true>), grid, dim3(Vr::bs), 0, st, a,
@@ -299,6 +350,15 @@ __global__ void __launch_bounds__(BS) k_xyplane(FieldOps<T> ops, int nt, const c
     xy_plane<T, NX, BS>(ops.f[f] + (size_t)t * NX * NX, ops.tab[f], ops.taby[f], tw, lds);
 }
 
+template <typename T, int NX, int BS>
+__global__ void __launch_bounds__(BS) k_xyplanep(FieldOps<T> ops, int nt, int nf,
+                                                 const cplx<T> *__restrict__ tw, const int *status) {
+    if (status && *status) return;
+    __shared__ cplx<T> lds[NX * (NX + 1)];
+    __shared__ cplx<T> stab[14 * NX];
+    xy_run<T, NX, BS>(XySolo<T, NX>{ops, nt, nf, tw}, lds, stab);
+}
+
 template <typename T, int NX>
 static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                               const cplx<T> *tw, const int *status) {
@@ -308,9 +368,19 @@ static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<
         constexpr int BS = NX >= 96 ? 512 : 256;
         hipLaunchKernelGGL((k_xyplane<T, NX, BS>), grid, dim3(BS), 0, st, ops, g.nt, tw, status);
     } else {
-        if (v == 0) hipLaunchKernelGGL((k_xyplane<T, NX, 256>), grid, dim3(256), 0, st, ops, g.nt, tw, status);
-        else if (v == 1) hipLaunchKernelGGL((k_xyplane<T, NX, 512>), grid, dim3(512), 0, st, ops, g.nt, tw, status);
-        else hipLaunchKernelGGL((k_xyplane<T, NX, 1024>), grid, dim3(1024), 0, st, ops, g.nt, tw, status);
+        const int bs = xy_variant_bs(v);
+        if (xy_variant_persistent(v)) {
+            const dim3 pg(persistent_grid(g.nt * nf, xy_variant_per_cu(v)));
+            if (bs == 256) hipLaunchKernelGGL((k_xyplanep<T, NX, 256>), pg, dim3(256), 0, st, ops, g.nt, nf, tw, status);
+            else if (bs == 512) hipLaunchKernelGGL((k_xyplanep<T, NX, 512>), pg, dim3(512), 0, st, ops, g.nt, nf, tw, status);
+            else hipLaunchKernelGGL((k_xyplanep<T, NX, 1024>), pg, dim3(1024), 0, st, ops, g.nt, nf, tw, status);
+        } else if (bs == 256) {
+            hipLaunchKernelGGL((k_xyplane<T, NX, 256>), grid, dim3(256), 0, st, ops, g.nt, tw, status);
+        } else if (bs == 512) {
+            hipLaunchKernelGGL((k_xyplane<T, NX, 512>), grid, dim3(512), 0, st, ops, g.nt, tw, status);
+        } else {
+            hipLaunchKernelGGL((k_xyplane<T, NX, 1024>), grid, dim3(1024), 0, st, ops, g.nt, tw, status);
+        }
     }
     return hipGetLastError();
 }

@@ -88,6 +88,10 @@ int tune_get(bool fp64, int axis, int n);
 bool tune_set(bool fp64, int axis, int n, int v);
 int tune_tile_px(bool fp64, int nt, int v);   // pixels per tile of a t-pass variant
 int tune_block(bool fp64, int axis, int n, int v);
+int tune_per_cu(bool fp64, int axis, int n, int v);
+// Grid of a persistent launch: per_cu workgroups per CU of the current
+// device, at most one per unit.
+unsigned persistent_grid(int units, int per_cu);  // persistent workgroups per CU, 0: one unit each
 
 bool tline_supported(int nt, bool fp64);
 bool xyplane_supported(int nx, int ny, bool fp64);

@@ -28,29 +28,71 @@ template <typename T, int NT> constexpr int tline3_tx() {
     return tx >= 16 ? 16 : (tx >= 8 ? 8 : (tx >= 4 ? 4 : (tx >= 2 ? 2 : 1)));
 }
 
-// Launch variants of the fused pass for one NT (chosen by the autotuner):
-// V = 2 m + b with tile width m = 0: TXd, 1: TXd/2, 2: 2 TXd pixels and
-// b = 0: 256, 1: 512 threads.  Only variant 0 exists for FP64 and for
-// NT < 64; a variant whose three tiles exceed 96 KiB of LDS does not exist.
-constexpr int kTimeVariants = 6;
+// Launch variants of the fused pass for one NT (chosen by the autotuner).
+// V < 6: one tile per workgroup, V = 2 m + b with tile width m = 0: TXd,
+// 1: TXd/2, 2: 2 TXd pixels and b = 0: 256, 1: 512 threads.
+// V >= 6: persistent workgroups (k + 1 per CU), each walking a contiguous
+// run of tiles with the next tile prefetched into registers;
+// V - 6 = 2 (3 m + b) + k with b = 0: 256, 1: 384, 2: 512 threads.
+// Only variant 0 exists for FP64 and for NT < 64; a variant whose three
+// tiles exceed 96 KiB of LDS (persistent: tiles plus staged tables beyond
+// 144 KiB) does not exist.
+constexpr int kTimeVariants = 24;
 template <typename T, int NT, int V> struct TimeVariant {
     static constexpr int txd = tline3_tx<T, NT>();
-    static constexpr int m = V / 2, b = V % 2;
+    static constexpr bool persistent = V >= 6;
+    static constexpr int idx = persistent ? V - 6 : V;
+    static constexpr int m = persistent ? idx / 6 : idx / 2;
+    static constexpr int b = persistent ? (idx / 2) % 3 : idx % 2;
+    static constexpr int per_cu = persistent ? idx % 2 + 1 : 0;
     static constexpr int tx = m == 0 ? txd : (m == 1 ? txd / 2 : txd * 2);
-    static constexpr int bs = b == 0 ? 256 : 512;
+    static constexpr int bs = persistent ? (b == 0 ? 256 : (b == 1 ? 384 : 512)) : (b == 0 ? 256 : 512);
+    static constexpr int data_bytes = 3 * NT * tx * (int)sizeof(cplx<T>);
+    static constexpr int table_bytes = 8 * NT * (int)sizeof(cplx<T>);  // twiddles + 3 tables, hi/lo
     static constexpr bool valid =
-        V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 &&
-                   3 * NT * tx * (int)sizeof(cplx<T>) <= 98304);
+        V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 98304 &&
+                   (!persistent || data_bytes + table_bytes <= 147456));
 };
 
-// Block-size variants of the xy pass: 256, 512 or 1024 threads.
-constexpr int kXyVariants = 3;
-constexpr int xy_variant_bs(int v) { return v == 0 ? 256 : (v == 1 ? 512 : 1024); }
+// Variants of the xy pass: V < 3 one plane per workgroup of 256, 512 or
+// 1024 threads; V >= 3 persistent workgroups (k + 1 per CU) walking a run
+// of planes with the next plane prefetched into registers,
+// V - 3 = 2 b + k with the same block sizes.  Persistent variants need at
+// most 16 prefetched elements per thread.
+constexpr int kXyVariants = 9;
+constexpr int xy_variant_bs(int v) {
+    const int b = v < 3 ? v : (v - 3) / 2;
+    return b == 0 ? 256 : (b == 1 ? 512 : 1024);
+}
+constexpr bool xy_variant_persistent(int v) { return v >= 3; }
+constexpr int xy_variant_per_cu(int v) { return v >= 3 ? (v - 3) % 2 + 1 : 0; }
 constexpr int xy_default_variant(int nx) { return nx >= 96 ? 1 : 0; }
 
 template <typename T> struct alignas(4 * sizeof(T)) cpair {
     cplx<T> a, b;
 };
+
+// Pointwise part of the fused pass for two cells: RK4 half step(s), then
+// the mirror factors on s and i when the pass closes a round trip.
+template <typename T, bool PHASE>
+__device__ __forceinline__ void nl_point_pair(const Rk4Args<T> &rk, Tri<T> &A0, Tri<T> &A1) {
+    if (rk.nsteps > 0) {
+        rk4_half_step<T, PHASE>(rk, 0, A0);
+        rk4_half_step<T, PHASE>(rk, 0, A1);
+        if (rk.nsteps > 1) {
+            rk4_half_step<T, PHASE>(rk, 1, A0);
+            rk4_half_step<T, PHASE>(rk, 1, A1);
+        }
+    }
+    if (rk.out_scale) {
+        const cplx<T> ms = {rk.ms_re, rk.ms_im}, msl = {rk.ms_lo_re, rk.ms_lo_im};
+        const cplx<T> mi = {rk.mi_re, rk.mi_im}, mil = {rk.mi_lo_re, rk.mi_lo_im};
+        A0.s = cmul_hl(A0.s, ms, msl);
+        A1.s = cmul_hl(A1.s, ms, msl);
+        A0.i = cmul_hl(A0.i, mi, mil);
+        A1.i = cmul_hl(A1.i, mi, mil);
+    }
+}
 
 // One tile of the fused pass: TX consecutive pixels of row y, all NT time
 // samples, all three fields.  lds holds 3 * NT * TX elements.
@@ -73,22 +115,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             cpair<T> s = *reinterpret_cast<const cpair<T> *>(a.f[1] + gi);
             cpair<T> q = *reinterpret_cast<const cpair<T> *>(a.f[2] + gi);
             Tri<T> A0 = {p.a, s.a, q.a}, A1 = {p.b, s.b, q.b};
-            if (rk.nsteps > 0) {
-                rk4_half_step<T, PHASE>(rk, 0, A0);
-                rk4_half_step<T, PHASE>(rk, 0, A1);
-                if (rk.nsteps > 1) {
-                    rk4_half_step<T, PHASE>(rk, 1, A0);
-                    rk4_half_step<T, PHASE>(rk, 1, A1);
-                }
-            }
-            if (rk.out_scale) {
-                const cplx<T> ms = {rk.ms_re, rk.ms_im}, msl = {rk.ms_lo_re, rk.ms_lo_im};
-                const cplx<T> mi = {rk.mi_re, rk.mi_im}, mil = {rk.mi_lo_re, rk.mi_lo_im};
-                A0.s = cmul_hl(A0.s, ms, msl);
-                A1.s = cmul_hl(A1.s, ms, msl);
-                A0.i = cmul_hl(A0.i, mi, mil);
-                A1.i = cmul_hl(A1.i, mi, mil);
-            }
+            nl_point_pair<T, PHASE>(rk, A0, A1);
             lds[i] = A0.p;
             lds[i + 1] = A1.p;
             lds[TILE + i] = A0.s;
@@ -134,6 +161,120 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
 }
 
 
+// Contiguous run [u0, u1) of the work units of this workgroup.
+__device__ __forceinline__ void unit_run(int units, int &u0, int &u1) {
+    const long long G = gridDim.x, b = blockIdx.x;
+    u0 = (int)(b * units / G);
+    u1 = (int)((b + 1) * units / G);
+}
+
+// Persistent form of the fused pass: the workgroup walks a run of tiles.
+// The raw fields of tile u+1 are loaded into registers as soon as tile u
+// is in LDS, so their HBM latency hides behind the line operator of tile
+// u; twiddles and tables are staged in LDS (stab, 8 NT elements) whenever
+// the tile belongs to another solve than the staged tables, so the
+// prefetch is the only global load in flight inside the loop.
+//
+// Src supplies the work units: units(), skip(u), key(u) (identity of the
+// tables a unit needs), args(u) (pointers, step weights, first field),
+// nx(u), ny(u), tile(u), and stage(u, stab, tid, nthr) (copy twiddles to
+// stab[0, 2NT) and the line tables of fields first..2 after them).
+template <typename T, int NT, int TX, int BS, typename Src>
+__device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T> *stab) {
+    static_assert(TX % 2 == 0, "pairs of pixels");
+    constexpr int TILE = NT * TX;
+    constexpr int PT = (TILE + 2 * BS - 1) / (2 * BS);
+    constexpr bool EVEN = TILE % (2 * BS) == 0;
+    const int tid = threadIdx.x;
+    int u0, u1;
+    unit_run(src.units(), u0, u1);
+    cpair<T> rp[PT], rs[PT], ri[PT];
+    auto offset = [&](int u, int i) {
+        const int nx = src.nx(u), tiles_x = nx / TX, tile = src.tile(u);
+        const size_t base = (size_t)(tile / tiles_x) * nx + (tile % tiles_x) * TX;
+        return (size_t)(i / TX) * ((size_t)nx * src.ny(u)) + base + i % TX;
+    };
+    auto load = [&](int u) {
+        if (src.skip(u)) return;
+        const TLine3Args<T> a = src.args(u);
+#pragma unroll
+        for (int j = 0; j < PT; ++j) {
+            const int i = 2 * tid + j * 2 * BS;
+            if (EVEN || i < TILE) {
+                const size_t gi = offset(u, i);
+                rp[j] = *reinterpret_cast<const cpair<T> *>(a.psrc + gi);
+                rs[j] = *reinterpret_cast<const cpair<T> *>(a.f[1] + gi);
+                ri[j] = *reinterpret_cast<const cpair<T> *>(a.f[2] + gi);
+            }
+        }
+    };
+    int staged = -1;
+    if (u0 < u1) load(u0);
+    for (int u = u0; u < u1; ++u) {
+        if (src.skip(u)) {
+            if (u + 1 < u1) load(u + 1);
+            continue;
+        }
+        const TLine3Args<T> a = src.args(u);
+        if (src.key(u) != staged) {   // the barrier below publishes the tables
+            src.stage(u, stab, tid, BS);
+            staged = src.key(u);
+        }
+#pragma unroll
+        for (int j = 0; j < PT; ++j) {
+            const int i = 2 * tid + j * 2 * BS;
+            if (EVEN || i < TILE) {
+                Tri<T> A0 = {rp[j].a, rs[j].a, ri[j].a}, A1 = {rp[j].b, rs[j].b, ri[j].b};
+                nl_point_pair<T, false>(a.rk, A0, A1);
+                lds[i] = A0.p;
+                lds[i + 1] = A1.p;
+                lds[TILE + i] = A0.s;
+                lds[TILE + i + 1] = A1.s;
+                lds[2 * TILE + i] = A0.i;
+                lds[2 * TILE + i + 1] = A1.i;
+            }
+        }
+        __syncthreads();
+        if (u + 1 < u1) load(u + 1);
+        const LineSet<T> ls{lds + a.first_field * TILE, TILE, stab + 2 * NT, 2 * NT, 3 - a.first_field};
+        line_op_set<T, NT, +1, 1, TX>(ls, TX, stab, tid, BS);
+        for (int i = 2 * tid; i < TILE; i += 2 * BS) {
+            const size_t gi = offset(u, i);
+            *reinterpret_cast<cpair<T> *>(a.f[0] + gi) = cpair<T>{lds[i], lds[i + 1]};
+            *reinterpret_cast<cpair<T> *>(a.f[1] + gi) = cpair<T>{lds[TILE + i], lds[TILE + i + 1]};
+            *reinterpret_cast<cpair<T> *>(a.f[2] + gi) = cpair<T>{lds[2 * TILE + i], lds[2 * TILE + i + 1]};
+        }
+        __syncthreads();
+    }
+}
+
+// Copy twiddles (2NT elements) and the line tables of fields first..2
+// (each 2NT elements, tab_stride apart) into stab.
+template <typename T, int NT>
+__device__ __forceinline__ void stage_time_tables(const cplx<T> *__restrict__ tw, const TLine3Args<T> &a,
+                                                  cplx<T> *stab, int tid, int nthr) {
+    for (int i = tid; i < 2 * NT; i += nthr) stab[i] = tw[i];
+    for (int f = 0; f < 3 - a.first_field; ++f)
+        for (int i = tid; i < 2 * NT; i += nthr) stab[2 * NT * (1 + f) + i] = a.tab0[(size_t)f * a.tab_stride + i];
+}
+
+// Work units of the fused pass of one solve: its tiles.
+template <typename T, int NT, int TX> struct TimeSolo {
+    const TLine3Args<T> &a;
+    int nx_, ny_;
+    const cplx<T> *tw;
+    __device__ int units() const { return (nx_ / TX) * ny_; }
+    __device__ bool skip(int) const { return false; }
+    __device__ int key(int) const { return 0; }
+    __device__ const TLine3Args<T> &args(int) const { return a; }
+    __device__ int nx(int) const { return nx_; }
+    __device__ int ny(int) const { return ny_; }
+    __device__ int tile(int u) const { return u; }
+    __device__ void stage(int, cplx<T> *stab, int tid, int nthr) const {
+        stage_time_tables<T, NT>(tw, a, stab, tid, nthr);
+    }
+};
+
 // Q_x then Q_y on the NX x NX plane A (x fastest).  lds holds NX*(NX+1)
 // elements: rows padded by one element so row and column walks spread over
 // the LDS banks.
@@ -151,5 +292,88 @@ __device__ __forceinline__ void xy_plane(cplx<T> *A, const cplx<T> *__restrict__
     line_op<T, NX, -1, 1, NXP>(lds, NX, tw, taby, tid, BS);
     for (int i = tid; i < NX * NX; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
 }
+
+// Persistent form of the xy pass: the workgroup walks a run of planes and
+// loads plane u+1 into registers while plane u is transformed.  stab holds
+// 14 NX staged elements: twiddles, then the x tables of three fields, then
+// their y tables.  Src supplies units(), skip(u), key(u), plane(u) (the
+// plane's first element), slot(u) (which staged table pair it uses) and
+// stage(u, stab, tid, nthr).
+template <typename T, int NX, int BS, typename Src>
+__device__ __forceinline__ void xy_run(const Src &src, cplx<T> *lds, cplx<T> *stab) {
+    constexpr int NXP = NX + 1, NN = NX * NX;
+    constexpr int PT = (NN + BS - 1) / BS;
+    constexpr bool EVEN = NN % BS == 0;
+    const int tid = threadIdx.x;
+    int u0, u1;
+    unit_run(src.units(), u0, u1);
+    cplx<T> r[PT];
+    auto load = [&](int u) {
+        if (src.skip(u)) return;
+        const cplx<T> *A = src.plane(u);
+#pragma unroll
+        for (int j = 0; j < PT; ++j) {
+            const int i = tid + j * BS;
+            if (EVEN || i < NN) r[j] = A[i];
+        }
+    };
+    int staged = -1;
+    if (u0 < u1) load(u0);
+    for (int u = u0; u < u1; ++u) {
+        if (src.skip(u)) {
+            if (u + 1 < u1) load(u + 1);
+            continue;
+        }
+        if (src.key(u) != staged) {
+            src.stage(u, stab, tid, BS);
+            staged = src.key(u);
+        }
+#pragma unroll
+        for (int j = 0; j < PT; ++j) {
+            const int i = tid + j * BS;
+            if (EVEN || i < NN) lds[(i / NX) * NXP + (i % NX)] = r[j];
+        }
+        __syncthreads();
+        if (u + 1 < u1) load(u + 1);
+        const int k = src.slot(u);
+        line_op<T, NX, -1, NXP, 1>(lds, NX, stab, stab + 2 * NX * (1 + k), tid, BS);
+        line_op<T, NX, -1, 1, NXP>(lds, NX, stab, stab + 2 * NX * (4 + k), tid, BS);
+        cplx<T> *A = src.plane(u);
+        for (int i = tid; i < NN; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
+        __syncthreads();
+    }
+}
+
+// Copy twiddles and nf pairs of (x, y) tables into stab.
+template <typename T, int NX>
+__device__ __forceinline__ void stage_xy_tables(const cplx<T> *__restrict__ tw, const cplx<T> *const *tx,
+                                                const cplx<T> *const *ty, int nf, cplx<T> *stab, int tid,
+                                                int nthr) {
+    for (int i = tid; i < 2 * NX; i += nthr) stab[i] = tw[i];
+    for (int f = 0; f < nf; ++f)
+        for (int i = tid; i < 2 * NX; i += nthr) {
+            stab[2 * NX * (1 + f) + i] = tx[f][i];
+            stab[2 * NX * (4 + f) + i] = ty[f][i];
+        }
+}
+
+// Work units of the xy pass of one solve: plane t of field f, u = f nt + t.
+template <typename T, int NX> struct XySolo {
+    const FieldOps<T> &ops;
+    int nt, nf;
+    const cplx<T> *tw;
+    __device__ int units() const { return nt * nf; }
+    __device__ bool skip(int) const { return false; }
+    __device__ int key(int) const { return 0; }
+    __device__ int slot(int u) const { return u / nt; }
+    __device__ cplx<T> *plane(int u) const {
+        const int f = u / nt;
+        cplx<T> *base = f == 0 ? ops.f[0] : (f == 1 ? ops.f[1] : ops.f[2]);
+        return base + (size_t)(u % nt) * NX * NX;
+    }
+    __device__ void stage(int, cplx<T> *stab, int tid, int nthr) const {
+        stage_xy_tables<T, NX>(tw, ops.tab, ops.taby, nf, stab, tid, nthr);
+    }
+};
 
 }  // namespace opo

@@ -27,3 +27,33 @@ def cavity_params(n, precision=OPO_FP32, **kw):
     ph = configs.physical(precision=precision, **kw)
     p = configs.make_params(grid, ph)
     return grid, p, ph
+
+
+def batch_mix(n=4, trips=3, seed=3):
+    """n cavity solves of uneven grids drawn like the sweep's parameters."""
+    out = []
+    draws = configs.sweep_draws(n_solves=16, seed=seed)
+    sizes = [(32, 128), (64, 64), (96, 128), (32, 256)]
+    for k in range(n):
+        d = dict(draws[k])
+        d["nxy"], d["nt"] = sizes[k % len(sizes)]
+        out.append(configs.sweep_solve(d, n_round_trips=trips))
+    return out
+
+
+def batch_run(solves, trips, budget=8 << 30):
+    """Submit the solves to one batch, step until all finish; results in order."""
+    import cuopo3d_amd as co
+    b = co.Batch(budget)
+    tickets = [b.submit(g, p, *f, trips) for (g, p, ph, f) in solves]
+    results = {}
+    for _ in range(10 * trips * len(solves) + 10):
+        b.step()
+        for t in tickets:
+            if t not in results:
+                r = b.poll(t)
+                if r is not None:
+                    results[t] = r
+        if len(results) == len(tickets):
+            break
+    return [results[t] for t in tickets], b

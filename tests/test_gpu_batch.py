@@ -14,7 +14,7 @@ from cuopo3d_amd import configs
 from cuopo3d_amd.capi import OPO_FP32
 from oracle.pyoracle import Oracle
 
-from helpers import rel_l2
+from helpers import rel_l2, batch_mix, batch_run
 
 pytestmark = pytest.mark.gpu
 
@@ -22,30 +22,11 @@ GB = 1 << 30
 
 
 def mix(n=4, trips=3, seed=3):
-    out = []
-    draws = configs.sweep_draws(n_solves=16, seed=seed)
-    sizes = [(32, 128), (64, 64), (96, 128), (32, 256)]
-    for k in range(n):
-        d = dict(draws[k])
-        d["nxy"], d["nt"] = sizes[k % len(sizes)]
-        out.append(configs.sweep_solve(d, n_round_trips=trips))
-    return out
+    return batch_mix(n, trips, seed)
 
 
 def run_and_collect(solves, trips, budget=8 * GB):
-    b = co.Batch(budget)
-    tickets = [b.submit(g, p, *f, trips) for (g, p, ph, f) in solves]
-    results = {}
-    for _ in range(10 * trips * len(solves) + 10):
-        b.step()
-        for t in tickets:
-            if t not in results:
-                r = b.poll(t)
-                if r is not None:
-                    results[t] = r
-        if len(results) == len(tickets):
-            break
-    return [results[t] for t in tickets], b
+    return batch_run(solves, trips, budget)
 
 
 def test_batch_alone_equals_mixed_bitwise(require_gpu):

@@ -11,7 +11,7 @@ import cuopo3d_amd as co
 from cuopo3d_amd import configs
 from cuopo3d_amd.capi import OPO_FP32, OPO_OK, OPO_TUNE_TIME_PASS, OPO_TUNE_XY_PASS
 
-from helpers import rel_l2
+from helpers import rel_l2, batch_mix, batch_run
 
 pytestmark = pytest.mark.gpu
 
@@ -35,7 +35,7 @@ def _valid(lib, axis, n):
     """Variants that exist for size n."""
     keep = lib.opo_tune_get(OPO_FP32, axis, n)
     out = []
-    for v in range(8):
+    for v in range(32):
         if lib.opo_tune_set(OPO_FP32, axis, n, v) == OPO_OK:
             out.append(v)
     lib.opo_tune_set(OPO_FP32, axis, n, keep)
@@ -95,3 +95,39 @@ def test_autotune_keeps_fields_and_picks_a_variant(restore_tuning, require_gpu):
     v = lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, 256)
     assert v in _valid(lib, OPO_TUNE_TIME_PASS, 256)
 
+
+
+def _persistent(lib, axis, n):
+    """Valid persistent variants of size n (time: V >= 6, xy: V >= 3)."""
+    first = 6 if axis == OPO_TUNE_TIME_PASS else 3
+    return [v for v in _valid(lib, axis, n) if v >= first]
+
+
+def test_batch_with_persistent_variants(restore_tuning, require_gpu):
+    """The batched passes on persistent launches give the fields of the
+    one-item-per-workgroup launches, bitwise, for every size class."""
+    lib = restore_tuning
+    solves = batch_mix(4, trips=2)
+    nts = sorted({g.nt for (g, _, _, _) in solves})
+    nxs = sorted({g.nx for (g, _, _, _) in solves})
+    for nt in nts:
+        lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, nt, 0)
+    for nx in nxs:
+        lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, nx, 0)
+    ref, _ = batch_run(solves, 2)
+    # two rounds of persistent choices: first and last valid of each class
+    for pick in (0, -1):
+        for nt in nts:
+            vs = _persistent(lib, OPO_TUNE_TIME_PASS, nt)
+            assert vs, nt
+            assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, nt, vs[pick]) == OPO_OK
+        for nx in nxs:
+            vs = _persistent(lib, OPO_TUNE_XY_PASS, nx)
+            if vs:
+                assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, nx, vs[pick]) == OPO_OK
+        got, _ = batch_run(solves, 2)
+        for k, ((pr, fr), (pg, fg)) in enumerate(zip(ref, got)):
+            for a, b in zip(fr, fg):
+                assert np.array_equal(a, b), (pick, k)
+            assert np.array_equal(pr, pg), (pick, k)
+        print("persistent batch == classic batch, pick", pick, flush=True)
