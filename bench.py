@@ -76,7 +76,7 @@ def main():
     S = p.n_slices
     s = co.Solver(grid, p, device=0)
     s.set_fields(*fields)
-    tune_report = s.autotune(reps=20) if not args.no_tune else "not tuned"
+    tune_report = s.autotune(reps=5) if not args.no_tune else "not tuned"
     s.runOPO(args.warmup)
     s.sync()   # all of the solve's work is on its stream
     t0 = time.perf_counter()
@@ -172,7 +172,8 @@ def sweep_leg(co, configs, trips, n_solves=64, plain_solves=8, plain_trips=1):
     API, and against the plain rocFFT path one solve at a time (first solves)."""
     draws = configs.sweep_draws(n_solves=n_solves, seed=3)
     solves = [configs.sweep_solve(d, n_round_trips=trips) for d in draws]
-    co.autotune_grids([(g.nx, g.ny, g.nt) for (g, _, _, _) in solves])
+    for line in co.autotune_grids([(g.nx, g.ny, g.nt) for (g, _, _, _) in solves], reps=3):
+        print("sweep autotune:", line.strip().replace("\n", " | "), file=sys.stderr)
     work = sum(g.nx * g.ny * g.nt * p.n_slices for (g, p, _, _) in solves)
     b = co.Batch(200 << 30)
     tickets = [b.submit(g, p, *f, trips) for (g, p, _, f) in solves]
@@ -226,7 +227,8 @@ def sweep_leg(co, configs, trips, n_solves=64, plain_solves=8, plain_trips=1):
 def serving_leg(n, rate):
     import cuopo3d_amd as co
     from cuopo3d_amd import serving, configs
-    co.autotune_grids([(nxy, nxy, nt) for nxy, nt in configs.SERVING_GRIDS])
+    for line in co.autotune_grids([(nxy, nxy, nt) for nxy, nt in configs.SERVING_GRIDS], reps=3):
+        print("serving autotune:", line.strip().replace("\n", " | "), file=sys.stderr)
     reqs = serving.poisson_requests(n, rate_per_s=rate, seed=5)
     rep = serving.serve(reqs, budget_bytes=128 << 30, keep_fields=False)
     d = rep.as_dict()

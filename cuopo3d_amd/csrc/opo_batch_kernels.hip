@@ -39,7 +39,7 @@ template <typename T, int NT, int TX, int BS>
 __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ sd,
                                               const int *__restrict__ status,
                                               const WorkItem *__restrict__ work, int mode) {
-    __shared__ cplx<T> lds[3 * NT * TX];
+    __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
     const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
     if (status[w.solve]) return;
     const SolveDesc<T> &d = sd[w.solve];
@@ -101,7 +101,7 @@ template <typename T, int NT, int TX, int BS>
 __global__ void __launch_bounds__(BS) kb_timep(const SolveDesc<T> *__restrict__ sd,
                                                const int *__restrict__ status,
                                                const WorkItem *__restrict__ work, int n_work, int mode) {
-    __shared__ cplx<T> lds[3 * NT * TX];
+    __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
     __shared__ cplx<T> stab[8 * NT];
     nl_time_run<T, NT, TX, BS>(TimeBatch<T, NT, TX>{sd, status, work, n_work, mode}, lds, stab);
 }
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) kb_power_partial(const SolveDesc<T> *__re
         double s = 0.0, c = 0.0;
         for (size_t i = lo + tid; i < hi; i += 256) {
             const double re = (double)A[i].x, im = (double)A[i].y;
-            neumaier(s, c, re * re + im * im);
+            neumaier(s, c, fma(re, re, im * im));
         }
         red[f][tid] = s + c;
     }

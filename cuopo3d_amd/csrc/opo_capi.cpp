@@ -620,36 +620,46 @@ template <typename T> struct SolveT final : SolveImpl {
         }
         CK(hipEventCreate(&e0));
         CK(hipEventCreate(&e1));
+        // Candidates are timed in context: a stretch of the slice schedule
+        // ([t pass][xy pass] per slice), so each pass runs after the other
+        // as it does in a round trip.  Coordinate descent: the t pass with
+        // the xy pass fixed, then the xy pass, then the t pass again.
         const Rk4Args<T> rk = rk_args(2, 0.0, p.dz, false);
+        const int L = p.n_slices < 1 ? 1 : (p.n_slices < 16 ? p.n_slices : 16);
+        auto stretch = [&]() {
+            int r = OPO_OK;
+            for (int k = 0; k < L && !r; ++k) {
+                r = q_nl_time(A[0], rk, tabT[0], (int)(tabT[1] - tabT[0]), 0, OPO_STAGE_TIME_OP, nullptr);
+                if (!r) r = q_diffraction(nullptr);
+            }
+            return r;
+        };
         char buf[160];
-        for (int axis = 0; axis < 2; ++axis) {
+        const int order[3] = {0, 1, 0};
+        for (int round = 0; round < 3; ++round) {
+            const int axis = order[round];
             const int n = axis == 0 ? g.nt : g.nx;
             int best = tune_get(fp64, axis, n);
             double best_t = 1e30;
             report += axis == 0 ? "t pass NT=" : "xy pass NX=";
-            report += std::to_string(n) + ":";
+            report += std::to_string(n) + " (us per slice):";
             for (int v = 0; v < tune_variant_count(axis); ++v) {
                 if (!tune_variant_valid(fp64, axis, n, v)) continue;
                 if (axis == 0 && (g.nx == 1 || g.nx % tune_tile_px(fp64, n, v) != 0)) continue;
                 tune_set(fp64, axis, n, v);
-                auto launch = [&]() {
-                    return axis == 0 ? q_nl_time(A[0], rk, tabT[0], (int)(tabT[1] - tabT[0]), 0,
-                                                 OPO_STAGE_TIME_OP, nullptr)
-                                     : q_diffraction(nullptr);
-                };
-                if ((rc = launch())) { cleanup(); return rc; }
+                if ((rc = stretch())) { cleanup(); return rc; }
                 CK(hipEventRecord(e0, stream));
-                for (int r = 0; r < reps && !rc; ++r) rc = launch();
+                for (int r = 0; r < reps && !rc; ++r) rc = stretch();
                 if (rc) { cleanup(); return rc; }
                 CK(hipEventRecord(e1, stream));
                 CK(hipEventSynchronize(e1));
                 float ms = 0.f;
                 CK(hipEventElapsedTime(&ms, e0, e1));
-                const double us = 1e3 * ms / reps;
+                const double us = 1e3 * ms / ((double)reps * L);
                 const int pc = tune_per_cu(fp64, axis, n, v);
                 char pers[32] = "";
                 if (pc) std::snprintf(pers, sizeof(pers), ", persistent %d/CU", pc);
-                std::snprintf(buf, sizeof(buf), " v%d(tile %d, block %d%s) %.2f us;", v,
+                std::snprintf(buf, sizeof(buf), " v%d(tile %d, block %d%s) %.2f;", v,
                               axis == 0 ? tune_tile_px(fp64, n, v) : n, tune_block(fp64, axis, n, v), pers, us);
                 report += buf;
                 if (us < best_t) {

@@ -26,6 +26,15 @@ template <typename T> struct alignas(2 * sizeof(T)) cplx {
     T x, y;
 };
 
+// Every multiply-add is written as an explicit fused operation and the
+// library is compiled with floating-point contraction off, so each result
+// is one fixed sequence of IEEE operations: tile shapes, block sizes and
+// launch forms cannot change a single bit of it.
+template <typename T> __device__ __forceinline__ T fma_t(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return fmaf(a, b, c);
+    else return fma(a, b, c);
+}
+
 template <typename T> __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
     return {a.x + b.x, a.y + b.y};
 }
@@ -34,11 +43,11 @@ template <typename T> __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T>
 }
 // (a.x b.x - a.y b.y, a.x b.y + a.y b.x): the product of Operators.cuh:109-116.
 template <typename T> __device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    return {fma_t(a.x, b.x, -(a.y * b.y)), fma_t(a.x, b.y, a.y * b.x)};
 }
 // a * conj(b)
 template <typename T> __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
-    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+    return {fma_t(a.x, b.x, a.y * b.y), fma_t(a.y, b.x, -(a.x * b.y))};
 }
 template <typename T> __device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) {
     return {a.x * s, a.y * s};
@@ -62,7 +71,7 @@ __device__ __forceinline__ cplx<T> cmul_hl(cplx<T> a, cplx<T> hi, cplx<T> lo) {
     if constexpr (sizeof(T) == 8) {
         return cmul(a, hi);
     } else {
-        const T cr = a.x * lo.x - a.y * lo.y, ci = a.x * lo.y + a.y * lo.x;
+        const T cr = fmaf(a.x, lo.x, -(a.y * lo.y)), ci = fmaf(a.x, lo.y, a.y * lo.x);
         return {fmaf(a.x, hi.x, fmaf(-a.y, hi.y, cr)), fmaf(a.x, hi.y, fmaf(a.y, hi.x, ci))};
     }
 }
@@ -72,7 +81,7 @@ __device__ __forceinline__ cplx<T> cmulc_hl(cplx<T> a, cplx<T> hi, cplx<T> lo) {
     if constexpr (sizeof(T) == 8) {
         return cmulc(a, hi);
     } else {
-        const T cr = a.x * lo.x + a.y * lo.y, ci = a.y * lo.x - a.x * lo.y;
+        const T cr = fmaf(a.x, lo.x, a.y * lo.y), ci = fmaf(a.y, lo.x, -(a.x * lo.y));
         return {fmaf(a.x, hi.x, fmaf(a.y, hi.y, cr)), fmaf(a.y, hi.x, fmaf(-a.x, hi.y, ci))};
     }
 }
@@ -184,7 +193,7 @@ template <int S, typename T> __device__ __forceinline__ void dft3(cplx<T> *v) {
     constexpr double r3 = 0.86602540378443864676372317075294;  // sqrt(3)/2
     constexpr T r3h = (T)r3, r3l = (T)(r3 - (double)r3h);
     cplx<T> a0 = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-    cplx<T> u = {a0.x - (T)0.5 * t1.x, a0.y - (T)0.5 * t1.y};
+    cplx<T> u = {fma_t((T)-0.5, t1.x, a0.x), fma_t((T)-0.5, t1.y, a0.y)};
     cplx<T> w = mul_si<S>(cmul_hl(t2, cplx<T>{r3h, (T)0}, cplx<T>{r3l, (T)0}));
     v[0] = cadd(a0, t1);
     v[1] = cadd(u, w);
@@ -274,10 +283,16 @@ template <typename T> struct LineSet {
     __device__ __forceinline__ const cplx<T> *tab(int f) const { return t0 + f * tstride; }
 };
 
-template <typename T, int N, int S, int LS, int ES>
+// PAD: extra elements after every N2 consecutive elements of a line
+// (element n sits at n*ES + (n / N2)*PAD), so that the N1 rows walked in
+// phase 2 start in different LDS banks when N2*ES is a multiple of the
+// bank count.
+template <typename T, int N, int S, int LS, int ES, int PAD = 0>
 __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
                                             const cplx<T> *__restrict__ tw, int tid, int nthr) {
     constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+    constexpr int RS = N2 * ES + PAD;   // stride between the N1 rows of N2 elements
+    static_assert(N2 > 1 || PAD == 0, "padding applies to two-level lines");
     if constexpr (N2 == 1) {
         for (int task = tid; task < nlines * ls.nf; task += nthr) {
             const int l = task % nlines, f = task / nlines;
@@ -301,7 +316,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
             cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
             cplx<T> v[N1];
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[n1 * N2 * ES];
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[n1 * RS];
             dft<N1, S>(v);
 #pragma unroll
             for (int k1 = 1; k1 < N1; ++k1) {
@@ -310,14 +325,14 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
                               : cmulc_hl(v[k1], tw[2 * j], tw[2 * j + 1]);
             }
 #pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) base[k1 * N2 * ES] = v[k1];
+            for (int k1 = 0; k1 < N1; ++k1) base[k1 * RS] = v[k1];
         }
         __syncthreads();
         // phase 2: task (l, k1, f)
         for (int task = tid; task < nlines * N1 * ls.nf; task += nthr) {
             const int l = task % nlines, r = task / nlines, k1 = r % N1, f = r / N1;
             const cplx<T> *tab = ls.tab(f);
-            cplx<T> *base = ls.base(f) + l * LS + k1 * N2 * ES;
+            cplx<T> *base = ls.base(f) + l * LS + k1 * RS;
             cplx<T> v[N2];
 #pragma unroll
             for (int n2 = 0; n2 < N2; ++n2) v[n2] = base[n2 * ES];
@@ -344,10 +359,10 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
             cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
             cplx<T> v[N1];
 #pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[k1 * N2 * ES];
+            for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[k1 * RS];
             dft<N1, -S>(v);
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) base[n1 * N2 * ES] = v[n1];
+            for (int n1 = 0; n1 < N1; ++n1) base[n1 * RS] = v[n1];
         }
         __syncthreads();
     }

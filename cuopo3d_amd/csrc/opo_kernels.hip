@@ -130,7 +130,7 @@ template <typename T, int NT, int TX, int BS, bool PHASE>
 __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
                                                const cplx<T> *__restrict__ tw, const int *status) {
     if (status && *status) return;
-    __shared__ cplx<T> lds[3 * NT * TX];
+    __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
     nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds);
 }
 
@@ -138,7 +138,7 @@ template <typename T, int NT, int TX, int BS>
 __global__ void __launch_bounds__(BS) k_tline3p(TLine3Args<T> a, int nx, int ny,
                                                 const cplx<T> *__restrict__ tw, const int *status) {
     if (status && *status) return;
-    __shared__ cplx<T> lds[3 * NT * TX];
+    __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
     __shared__ cplx<T> stab[8 * NT];
     nl_time_run<T, NT, TX, BS>(TimeSolo<T, NT, TX>{a, nx, ny, tw}, lds, stab);
 }
@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(256) k_power_partial(const cplx<T> *f0, const 
             for (size_t i = (size_t)blockIdx.x * blockDim.x + tid; i < n; i += stride) {
                 cplx<T> a = F[f][i];
                 double re = (double)a.x, im = (double)a.y;
-                neumaier_add(s, c, re * re + im * im);
+                neumaier_add(s, c, fma(re, re, im * im));
             }
         }
         red[f][tid] = s + c;

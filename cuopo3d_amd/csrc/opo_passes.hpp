@@ -28,6 +28,16 @@ template <typename T, int NT> constexpr int tline3_tx() {
     return tx >= 16 ? 16 : (tx >= 8 ? 8 : (tx >= 4 ? 4 : (tx >= 2 ? 2 : 1)));
 }
 
+// LDS layout of one field's tile in the fused pass: time row t (TX
+// pixels) starts at t*TX + (t / N2)*PAD.  Narrow tiles pad every N2 rows by
+// TX elements so the N1 rows of the second phase fall in different banks.
+template <typename T, int NT, int TX> struct TimeTile {
+    static constexpr int N1 = Split<NT>::N1, N2 = Split<NT>::N2;
+    static constexpr int PAD = (N2 > 1 && TX < 16) ? TX : 0;
+    static constexpr int FT = NT * TX + N1 * PAD;   // elements per field
+    __device__ static __forceinline__ int at(int i) { return i + (i / (N2 * TX)) * PAD; }
+};
+
 // Launch variants of the fused pass for one NT (chosen by the autotuner).
 // V < 6: one tile per workgroup, V = 2 m + b with tile width m = 0: TXd,
 // 1: TXd/2, 2: 2 TXd pixels and b = 0: 256, 1: 512 threads.
@@ -35,7 +45,7 @@ template <typename T, int NT> constexpr int tline3_tx() {
 // run of tiles with the next tile prefetched into registers;
 // V - 6 = 2 (3 m + b) + k with b = 0: 256, 1: 384, 2: 512 threads.
 // Only variant 0 exists for FP64 and for NT < 64; a variant whose three
-// tiles exceed 96 KiB of LDS (persistent: tiles plus staged tables beyond
+// tiles exceed 112 KiB of LDS (persistent: tiles plus staged tables beyond
 // 144 KiB) does not exist.
 constexpr int kTimeVariants = 24;
 template <typename T, int NT, int V> struct TimeVariant {
@@ -47,10 +57,10 @@ template <typename T, int NT, int V> struct TimeVariant {
     static constexpr int per_cu = persistent ? idx % 2 + 1 : 0;
     static constexpr int tx = m == 0 ? txd : (m == 1 ? txd / 2 : txd * 2);
     static constexpr int bs = persistent ? (b == 0 ? 256 : (b == 1 ? 384 : 512)) : (b == 0 ? 256 : 512);
-    static constexpr int data_bytes = 3 * NT * tx * (int)sizeof(cplx<T>);
+    static constexpr int data_bytes = 3 * TimeTile<T, NT, (tx > 0 ? tx : 1)>::FT * (int)sizeof(cplx<T>);
     static constexpr int table_bytes = 8 * NT * (int)sizeof(cplx<T>);  // twiddles + 3 tables, hi/lo
     static constexpr bool valid =
-        V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 98304 &&
+        V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 114688 &&
                    (!persistent || data_bytes + table_bytes <= 147456));
 };
 
@@ -99,7 +109,8 @@ __device__ __forceinline__ void nl_point_pair(const Rk4Args<T> &rk, Tri<T> &A0, 
 template <typename T, int NT, int TX, int BS, bool PHASE>
 __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int ny, int tile,
                                              const cplx<T> *__restrict__ tw, cplx<T> *lds) {
-    constexpr int TILE = NT * TX;
+    using L = TimeTile<T, NT, TX>;
+    constexpr int TILE = NT * TX, FT = L::FT;
     const int tid = threadIdx.x;
     const int tiles_x = nx / TX;
     const int y = tile / tiles_x, x0 = (tile % tiles_x) * TX;
@@ -116,12 +127,13 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             cpair<T> q = *reinterpret_cast<const cpair<T> *>(a.f[2] + gi);
             Tri<T> A0 = {p.a, s.a, q.a}, A1 = {p.b, s.b, q.b};
             nl_point_pair<T, PHASE>(rk, A0, A1);
-            lds[i] = A0.p;
-            lds[i + 1] = A1.p;
-            lds[TILE + i] = A0.s;
-            lds[TILE + i + 1] = A1.s;
-            lds[2 * TILE + i] = A0.i;
-            lds[2 * TILE + i + 1] = A1.i;
+            const int li = L::at(i);
+            lds[li] = A0.p;
+            lds[li + 1] = A1.p;
+            lds[FT + li] = A0.s;
+            lds[FT + li + 1] = A1.s;
+            lds[2 * FT + li] = A0.i;
+            lds[2 * FT + li + 1] = A1.i;
         }
     } else {
         for (int i = tid; i < TILE; i += BS) {
@@ -134,28 +146,29 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
                 A.s = cmul_hl(A.s, cplx<T>{rk.ms_re, rk.ms_im}, cplx<T>{rk.ms_lo_re, rk.ms_lo_im});
                 A.i = cmul_hl(A.i, cplx<T>{rk.mi_re, rk.mi_im}, cplx<T>{rk.mi_lo_re, rk.mi_lo_im});
             }
-            lds[i] = A.p;
-            lds[TILE + i] = A.s;
-            lds[2 * TILE + i] = A.i;
+            const int li = L::at(i);
+            lds[li] = A.p;
+            lds[FT + li] = A.s;
+            lds[2 * FT + li] = A.i;
         }
     }
     __syncthreads();
-    const LineSet<T> ls{lds + a.first_field * TILE, TILE, a.tab0, a.tab_stride, 3 - a.first_field};
-    line_op_set<T, NT, +1, 1, TX>(ls, TX, tw, tid, BS);
+    const LineSet<T> ls{lds + a.first_field * FT, FT, a.tab0, a.tab_stride, 3 - a.first_field};
+    line_op_set<T, NT, +1, 1, TX, L::PAD>(ls, TX, tw, tid, BS);
     if constexpr (TX % 2 == 0) {
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
-            *reinterpret_cast<cpair<T> *>(a.f[0] + gi) = cpair<T>{lds[i], lds[i + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[1] + gi) = cpair<T>{lds[TILE + i], lds[TILE + i + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[2] + gi) =
-                cpair<T>{lds[2 * TILE + i], lds[2 * TILE + i + 1]};
+            const int li = L::at(i);
+            *reinterpret_cast<cpair<T> *>(a.f[0] + gi) = cpair<T>{lds[li], lds[li + 1]};
+            *reinterpret_cast<cpair<T> *>(a.f[1] + gi) = cpair<T>{lds[FT + li], lds[FT + li + 1]};
+            *reinterpret_cast<cpair<T> *>(a.f[2] + gi) = cpair<T>{lds[2 * FT + li], lds[2 * FT + li + 1]};
         }
     } else {
         for (int i = tid; i < TILE; i += BS) {
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
-            for (int f = 0; f < 3; ++f) a.f[f][gi] = lds[f * TILE + i];
+            for (int f = 0; f < 3; ++f) a.f[f][gi] = lds[f * FT + L::at(i)];
         }
     }
 }
@@ -182,7 +195,8 @@ __device__ __forceinline__ void unit_run(int units, int &u0, int &u1) {
 template <typename T, int NT, int TX, int BS, typename Src>
 __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T> *stab) {
     static_assert(TX % 2 == 0, "pairs of pixels");
-    constexpr int TILE = NT * TX;
+    using L = TimeTile<T, NT, TX>;
+    constexpr int TILE = NT * TX, FT = L::FT;
     constexpr int PT = (TILE + 2 * BS - 1) / (2 * BS);
     constexpr bool EVEN = TILE % (2 * BS) == 0;
     const int tid = threadIdx.x;
@@ -226,23 +240,25 @@ __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T
             if (EVEN || i < TILE) {
                 Tri<T> A0 = {rp[j].a, rs[j].a, ri[j].a}, A1 = {rp[j].b, rs[j].b, ri[j].b};
                 nl_point_pair<T, false>(a.rk, A0, A1);
-                lds[i] = A0.p;
-                lds[i + 1] = A1.p;
-                lds[TILE + i] = A0.s;
-                lds[TILE + i + 1] = A1.s;
-                lds[2 * TILE + i] = A0.i;
-                lds[2 * TILE + i + 1] = A1.i;
+                const int li = L::at(i);
+                lds[li] = A0.p;
+                lds[li + 1] = A1.p;
+                lds[FT + li] = A0.s;
+                lds[FT + li + 1] = A1.s;
+                lds[2 * FT + li] = A0.i;
+                lds[2 * FT + li + 1] = A1.i;
             }
         }
         __syncthreads();
         if (u + 1 < u1) load(u + 1);
-        const LineSet<T> ls{lds + a.first_field * TILE, TILE, stab + 2 * NT, 2 * NT, 3 - a.first_field};
-        line_op_set<T, NT, +1, 1, TX>(ls, TX, stab, tid, BS);
+        const LineSet<T> ls{lds + a.first_field * FT, FT, stab + 2 * NT, 2 * NT, 3 - a.first_field};
+        line_op_set<T, NT, +1, 1, TX, L::PAD>(ls, TX, stab, tid, BS);
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const size_t gi = offset(u, i);
-            *reinterpret_cast<cpair<T> *>(a.f[0] + gi) = cpair<T>{lds[i], lds[i + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[1] + gi) = cpair<T>{lds[TILE + i], lds[TILE + i + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[2] + gi) = cpair<T>{lds[2 * TILE + i], lds[2 * TILE + i + 1]};
+            const int li = L::at(i);
+            *reinterpret_cast<cpair<T> *>(a.f[0] + gi) = cpair<T>{lds[li], lds[li + 1]};
+            *reinterpret_cast<cpair<T> *>(a.f[1] + gi) = cpair<T>{lds[FT + li], lds[FT + li + 1]};
+            *reinterpret_cast<cpair<T> *>(a.f[2] + gi) = cpair<T>{lds[2 * FT + li], lds[2 * FT + li + 1]};
         }
         __syncthreads();
     }

@@ -36,7 +36,7 @@ __device__ __forceinline__ Tri<T> slopes(const Rk4Args<T> &a, int st, int stage,
 
 // a + i w q for a real weight w
 template <typename T> __device__ __forceinline__ cplx<T> add_iw(cplx<T> a, cplx<T> q, T w) {
-    return {a.x - w * q.y, a.y + w * q.x};
+    return {fma_t(-w, q.y, a.x), fma_t(w, q.x, a.y)};
 }
 
 // kernelLinealCombination (Solver3D.cuh:386-402): the stage point A + c k
@@ -52,8 +52,9 @@ template <typename T>
 __device__ __forceinline__ cplx<T> rk4_comb(cplx<T> a, cplx<T> q1, cplx<T> q2, cplx<T> q3, cplx<T> q4,
                                             T w) {
     const T two = (T)2;
-    cplx<T> s = cadd(cadd(cadd(q1, cscale(q2, two)), cscale(q3, two)), q4);
-    return add_iw(a, s, w);
+    cplx<T> s = {fma_t(two, q2.x, q1.x), fma_t(two, q2.y, q1.y)};
+    s = {fma_t(two, q3.x, s.x), fma_t(two, q3.y, s.y)};
+    return add_iw(a, cadd(s, q4), w);
 }
 
 template <typename T, bool PHASE>

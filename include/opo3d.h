@@ -197,11 +197,14 @@ int opo_profile_trips(opo_solve *s, int n, double ms[OPO_N_STAGES],
 /* Launch-variant autotuner (no counterpart in the reference, whose launch
  * shapes are compile-time constants, DtypesConsts.cuh:45-48).  For the
  * solve's grid and precision, times every compiled variant of the fused
- * t pass (tile width x block size) and of the xy pass (block size), `reps`
- * launches each, and keeps the fastest for all later launches of that size
- * in this process.  The solve's fields are saved and restored.  Results do
- * not depend on the variant (tiles only group lines).  report (nullable)
- * receives a one-line summary per pass. */
+ * t pass (tile width x block size, one tile per workgroup or persistent
+ * workgroups) and of the xy pass (block size, persistent or not) on a
+ * stretch of up to 16 slices of the real schedule, `reps` stretches each,
+ * by coordinate descent (t, xy, t), and keeps the fastest for all later
+ * launches of that size in this process.  The solve's fields are saved and
+ * restored.  Results are bitwise independent of the variant (tiles only
+ * group lines; every multiply-add is explicit).  report (nullable)
+ * receives one line per descent step. */
 #define OPO_TUNE_TIME_PASS 0
 #define OPO_TUNE_XY_PASS   1
 int opo_autotune(opo_solve *s, int reps, char *report, size_t report_len);

@@ -126,3 +126,28 @@ def test_batch_evicts_nan_solve_only(require_gpu):
     for t, ref in zip((tickets[0], tickets[2]), clean):
         for a, c in zip(out[t][1], ref[1]):
             assert np.array_equal(a, c)
+
+
+def test_sweep_cli_matches_single_solves(tmp_path, require_gpu):
+    """cuopo3d_amd.sweep over 2 powers x 2 waists: every point's per-trip
+    power equals the same point run alone through the single-solve API."""
+    from cuopo3d_amd import sweep
+    from cuopo3d_amd.capi import Grid
+    rc = sweep.main(["--powers", "500:600:100", "--waists", "31,40", "--grid", "32", "64",
+                     "--trips", "3", "--out", str(tmp_path)])
+    assert rc == 0
+    grid = Grid(32, 32, 64)
+    ph = sweep.reference_physical(3)
+    for w in (31, 40):
+        for pw in (500, 600):
+            folder = tmp_path / f"waist_{w}" / f"sPPLT_N_{pw}"
+            rows = np.loadtxt(folder / "power_per_trip.dat")
+            assert rows.shape == (3, 4)
+            assert (folder / f"OutputSignalPower_PumpPower{pw}_W_slice32_r.dat").exists()
+            assert (folder / "As_temporal_x_16_y_16_i.dat").exists()
+            p, f = sweep.point(grid, ph, pw, w)
+            s = co.Solver(grid, p, device=0)
+            s.set_fields(*f)
+            ref = s.runOPO(3)
+            # the file holds 17 significant digits of the batch's doubles
+            assert np.allclose(rows[:, 1:], ref, rtol=1e-15, atol=0), (w, pw)

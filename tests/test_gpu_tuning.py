@@ -1,8 +1,10 @@
 """Launch variants and the autotuner (opo_autotune, opo_tune_get/set).
 
 A variant changes only how lines are grouped into workgroups (tile width,
-block size), never the arithmetic of a line, so every variant must give the
-same fields; the tuner must leave the solve's fields as it found them.
+block size, persistent walk), never the arithmetic of a line, and every
+multiply-add is an explicit fused operation (no compiler contraction), so
+every variant must give bitwise the same fields; the tuner must leave the
+solve's fields as it found them.
 """
 import numpy as np
 import pytest
@@ -68,8 +70,8 @@ def test_variants_give_the_same_fields(n, restore_tuning, require_gpu):
             errs = [rel_l2(a, b) for a, b in zip(got, ref)]
             same = all(np.array_equal(a, b) for a, b in zip(got, ref))
             print(f"{n} axis {axis} v{v}: bitwise {same}, rel L2 {max(errs):.1e}", flush=True)
-            assert max(errs) < 1e-6
-            assert np.max(np.abs(pw - pref) / pref) < 1e-6
+            assert same, (axis, v, errs)
+            assert np.array_equal(pw, pref), (axis, v)
         lib.opo_tune_set(OPO_FP32, axis, size, 0)
 
 
@@ -83,7 +85,7 @@ def test_autotune_keeps_fields_and_picks_a_variant(restore_tuning, require_gpu):
         before = s.get_fields()
         report = s.autotune(reps=3)
         print(report, flush=True)
-        assert "t pass NT=256" in report and "xy pass NX=64" in report
+        assert "t pass NT=256" in report and "xy pass NX=64" in report, report
         after = s.get_fields()
         for a, b in zip(before, after):
             assert np.array_equal(a, b)
@@ -91,7 +93,7 @@ def test_autotune_keeps_fields_and_picks_a_variant(restore_tuning, require_gpu):
         s.runOPO(2)
         got = s.get_fields()
     for a, b in zip(got, ref):
-        assert rel_l2(a, b) < 1e-6
+        assert np.array_equal(a, b)
     v = lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, 256)
     assert v in _valid(lib, OPO_TUNE_TIME_PASS, 256)
 
