@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-trips", type=int, default=4)
+    ap.add_argument("--cpu-trips", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-trips", type=int, default=20)
     ap.add_argument("--no-plain", action="store_true", help="skip the rocFFT comparator")
@@ -45,6 +45,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=16, help="0 skips the serving run")
     ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
+    ap.add_argument("--tune-reps", type=int, default=5, help="timed stretches per tuning candidate")
+    ap.add_argument("--grid", type=int, nargs=2, default=(64, 256), metavar=("NXY", "NT"),
+                    help="grid of the cavity workload (default 64 256); other grids are for "
+                         "profiling and skip the CPU and comparator legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 128 MiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
@@ -71,12 +75,17 @@ def main():
         cs.set_fields(np.zeros((1024, 128, 128), np.complex64), None, None)
         cs.sync()
         cs.close()
-    grid, p, ph, fields = configs.cavity_64(n_round_trips=args.warmup + args.steps)
+    default_grid = tuple(args.grid) == (64, 256)
+    if not default_grid:
+        args.no_cpu = args.no_plain = True
+        args.sweep_trips = args.serve_requests = 0
+    n = (args.grid[0], args.grid[0], args.grid[1])
+    grid, p, ph, fields = configs.cavity_64(n=n, n_round_trips=args.warmup + args.steps)
     cells = grid.nx * grid.ny * grid.nt
     S = p.n_slices
     s = co.Solver(grid, p, device=0)
     s.set_fields(*fields)
-    tune_report = s.autotune(reps=5) if not args.no_tune else "not tuned"
+    tune_report = s.autotune(reps=args.tune_reps) if not args.no_tune else "not tuned"
     s.runOPO(args.warmup)
     s.sync()   # all of the solve's work is on its stream
     t0 = time.perf_counter()
@@ -102,7 +111,7 @@ def main():
     d = stages[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBs"], 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBs"] / HBM_PEAK_GBS, 4),
-                "traffic": None}
+                "traffic": measured_traffic(grid, dom)}
     gpu_kernel_ms = sum(v["total_ms"] for v in stages.values()) / args.profile_trips
 
     out = {
@@ -110,8 +119,9 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64 (f32)",
         "data": "synthetic (focused Gaussian pump + seeded Philox noise)",
-        "config": {"workload": "singly-resonant cavity 64x64x256, MgO:PPLN 5 mm, 100 slices/trip, "
-                               "Rs=0.98, GDD, free space; step = one round trip",
+        "config": {"workload": f"singly-resonant cavity {grid.nx}x{grid.ny}x{grid.nt}, MgO:PPLN 5 mm, "
+                               "100 slices/trip, window = round-trip time, Rs=0.98, idler not resonant "
+                               "(multiplier 1), GDD, free space; step = one round trip",
                    "grid": [grid.nx, grid.ny, grid.nt], "slices": S},
         "roofline": roofline,
         "stages": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
@@ -153,6 +163,20 @@ def main():
     if args.serve_requests > 0:
         out["serving"] = serving_leg(args.serve_requests, args.serve_rate)
     print(json.dumps(out))
+
+
+def measured_traffic(grid, stage):
+    """HBM bytes per launch of a stage's kernel from the committed counter
+    summary (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
+    128 MiB device copy), or None when no summary covers this grid."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"counters_cavity_{grid.nx}x{grid.ny}x{grid.nt}.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:.*\b" + re.escape(stage) + r"=(\d+)", text)
+    return int(m.group(1)) if m else None
 
 
 def plain_leg(trips=4):

@@ -181,7 +181,7 @@ static int nx_index(int nx) {
 static void tune_init() {
     if (g_tune_init) return;
     for (int f = 0; f < 2; ++f) {
-        for (int k = 0; k < 11; ++k) g_time_var[f][k] = 0;
+        for (int k = 0; k < 11; ++k) g_time_var[f][k] = time_default_variant(f != 0, 1 << k);
         const int nxs[6] = {1, 16, 32, 64, 96, 128};
         for (int k = 0; k < 6; ++k) g_xy_var[f][k] = xy_default_variant(nxs[k]);
     }

@@ -39,30 +39,34 @@ template <typename T, int NT, int TX> struct TimeTile {
 };
 
 // Launch variants of the fused pass for one NT (chosen by the autotuner).
-// V < 6: one tile per workgroup, V = 2 m + b with tile width m = 0: TXd,
-// 1: TXd/2, 2: 2 TXd pixels and b = 0: 256, 1: 512 threads.
-// V >= 6: persistent workgroups (k + 1 per CU), each walking a contiguous
-// run of tiles with the next tile prefetched into registers;
-// V - 6 = 2 (3 m + b) + k with b = 0: 256, 1: 384, 2: 512 threads.
+// V < 9: one tile per workgroup, V = 3 m + b with tile width m = 0: TXd,
+// 1: TXd/2, 2: 2 TXd pixels and b = 0: 256, 1: 512, 2: 384 threads (384
+// gives every lane a task in the line-operator phases of 4- and 8-pixel
+// tiles).  V >= 9: persistent workgroups (k + 1 per CU), each walking a
+// contiguous run of tiles with the next tile prefetched into registers;
+// V - 9 = 2 (3 m + b) + k with b = 0: 256, 1: 384, 2: 512 threads.
 // Only variant 0 exists for FP64 and for NT < 64; a variant whose three
 // tiles exceed 112 KiB of LDS (persistent: tiles plus staged tables beyond
 // 144 KiB) does not exist.
-constexpr int kTimeVariants = 24;
+constexpr int kTimeClassic = 9;
+constexpr int kTimeVariants = kTimeClassic + 18;
 template <typename T, int NT, int V> struct TimeVariant {
     static constexpr int txd = tline3_tx<T, NT>();
-    static constexpr bool persistent = V >= 6;
-    static constexpr int idx = persistent ? V - 6 : V;
-    static constexpr int m = persistent ? idx / 6 : idx / 2;
-    static constexpr int b = persistent ? (idx / 2) % 3 : idx % 2;
+    static constexpr bool persistent = V >= kTimeClassic;
+    static constexpr int idx = persistent ? V - kTimeClassic : V;
+    static constexpr int m = persistent ? idx / 6 : idx / 3;
+    static constexpr int b = persistent ? (idx / 2) % 3 : idx % 3;
     static constexpr int per_cu = persistent ? idx % 2 + 1 : 0;
     static constexpr int tx = m == 0 ? txd : (m == 1 ? txd / 2 : txd * 2);
-    static constexpr int bs = persistent ? (b == 0 ? 256 : (b == 1 ? 384 : 512)) : (b == 0 ? 256 : 512);
+    static constexpr int bs = persistent ? (b == 0 ? 256 : (b == 1 ? 384 : 512)) : (b == 0 ? 256 : (b == 1 ? 512 : 384));
     static constexpr int data_bytes = 3 * TimeTile<T, NT, (tx > 0 ? tx : 1)>::FT * (int)sizeof(cplx<T>);
     static constexpr int table_bytes = 8 * NT * (int)sizeof(cplx<T>);  // twiddles + 3 tables, hi/lo
     static constexpr bool valid =
         V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 114688 &&
                    (!persistent || data_bytes + table_bytes <= 147456));
 };
+// Untuned choice: the default tile with 512 threads where it exists.
+constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 64) ? 1 : 0; }
 
 // Variants of the xy pass: V < 3 one plane per workgroup of 256, 512 or
 // 1024 threads; V >= 3 persistent workgroups (k + 1 per CU) walking a run

@@ -100,8 +100,8 @@ def test_autotune_keeps_fields_and_picks_a_variant(restore_tuning, require_gpu):
 
 
 def _persistent(lib, axis, n):
-    """Valid persistent variants of size n (time: V >= 6, xy: V >= 3)."""
-    first = 6 if axis == OPO_TUNE_TIME_PASS else 3
+    """Valid persistent variants of size n (time: V >= 9, xy: V >= 3)."""
+    first = 9 if axis == OPO_TUNE_TIME_PASS else 3
     return [v for v in _valid(lib, axis, n) if v >= first]
 
 
