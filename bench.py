@@ -131,6 +131,8 @@ def main():
         "signal_power_last_trip_W": float(pw[-1, 1]),
     }
 
+    if default_grid:
+        out["idler_dumped"] = idler_dumped_leg(co, configs, args.steps)
     if not args.no_cpu:
         from oracle.pyoracle import Oracle, get_threads
         g2, p2, _, f2 = configs.cavity_64(n_round_trips=args.cpu_trips)
@@ -163,6 +165,24 @@ def main():
     if args.serve_requests > 0:
         out["serving"] = serving_leg(args.serve_requests, args.serve_rate)
     print(json.dumps(out))
+
+
+def idler_dumped_leg(co, configs, trips):
+    """The same cavity with the non-resonant idler removed at the mirror
+    (multiplier 0) instead of the reference's multiplier 1."""
+    g, p, _, f = configs.cavity_64(n_round_trips=trips + 1)
+    p.fb_re[1], p.fb_im[1] = 0.0, 0.0
+    s = co.Solver(g, p, device=0)
+    s.set_fields(*f)
+    s.runOPO(1)
+    s.sync()
+    t0 = time.perf_counter()
+    pw = s.runOPO(trips)
+    s.sync()
+    dt = time.perf_counter() - t0
+    s.close()
+    return {"trips": trips, "ms_per_step": dt / trips * 1e3,
+            "signal_power_last_trip_W": float(pw[-1, 1]), "idler_power_last_trip_W": float(pw[-1, 2])}
 
 
 def measured_traffic(grid, stage):

@@ -85,6 +85,7 @@ class Physical(C.Structure):
         ("n_round_trips", C.c_int),
         ("t_window", C.c_double),
         ("precision", C.c_int),
+        ("pol", C.c_char * 4),
     ]
 
 
@@ -154,6 +155,8 @@ def load():
         "opo_slice_count": (ip, [C.c_double, ip]),
         "opo_crystal_properties": (ip, [ip, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.POINTER(Crystal)]),
+        "opo_crystal_properties_pol": (ip, [ip, C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, C.c_char_p, C.POINTER(Crystal)]),
         "opo_setup_params": (ip, [C.POINTER(Grid), C.POINTER(Physical), C.POINTER(Params)]),
         "opo_setup_pump": (ip, [C.POINTER(Grid), C.POINTER(Params), C.POINTER(Physical),
                                 ip, C.c_double, C.c_double, C.c_double, C.c_double, ip, vp]),

@@ -28,6 +28,7 @@ from .capi import Grid, Params, Physical, OPO_FP32, OPO_FP64, check, load
 CRYSTAL_MGOPPLN = 0
 CRYSTAL_PPLN = 1
 CRYSTAL_MGOSPPLT = 2
+CRYSTAL_ZGP = 3
 
 PUMP_WAVEPLANE_CW = 0
 PUMP_WAVEPLANE_PULSED = 1

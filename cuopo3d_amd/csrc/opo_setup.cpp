@@ -88,6 +88,34 @@ Sellmeier bruner(double T) {
     return s;
 }
 
+// Zelmon model of ZnGeP2 (ZGP.cuh:101-125): n^2 = A + B L^2/(L^2 - C)
+// + D L^2/(L^2 - E) per polarisation.  With g = B C L/(L^2-C)^2
+// + D E L/(L^2-E)^2: dn/dL = -g/n and d2n/dL2 = (-g' - (dn/dL)^2)/n, the
+// closed forms the reference evaluates (ZGP.cuh:183-244).
+struct Zelmon {
+    double A, B, Cc, D, E;
+    double n(double L) const {
+        const double L2 = L * L;
+        return std::sqrt(A + B * L2 / (L2 - Cc) + D * L2 / (L2 - E));
+    }
+    double dn(double L) const {
+        const double L2 = L * L, u = L2 - Cc, v = L2 - E;
+        return -(B * Cc * L / (u * u) + D * E * L / (v * v)) / n(L);
+    }
+    double d2n(double L) const {
+        const double L2 = L * L, u = L2 - Cc, v = L2 - E, d1 = dn(L);
+        const double mg1 = B * Cc * (3 * L2 + Cc) / (u * u * u) + D * E * (3 * L2 + E) / (v * v * v);
+        return (mg1 - d1 * d1) / n(L);
+    }
+    double gv(double L) const { return kC / (n(L) - L * dn(L)); }
+    double gvd(double L) const { return L * L * L * d2n(L) / (2 * kPi * kC * kC); }
+};
+
+Zelmon zelmon(char pol) {
+    if (pol == 'o') return Zelmon{8.0409, 1.68625, 0.40824, 1.2880, 611.05};
+    return Zelmon{8.0929, 1.8649, 0.41468, 0.84052, 452.05};
+}
+
 // ------------------------------------------------------ Philox4x32-10
 struct Philox {
     uint32_t k0, k1;
@@ -131,7 +159,35 @@ int opo_slice_count(double lcr, int nz) {
 
 int opo_crystal_properties(int crystal, double T, double Lambda, double lp, double ls, double li,
                            opo_crystal_props *out) {
+    return opo_crystal_properties_pol(crystal, T, Lambda, lp, ls, li, nullptr, out);
+}
+
+int opo_crystal_properties_pol(int crystal, double T, double Lambda, double lp, double ls, double li,
+                               const char *pol, opo_crystal_props *out) {
     if (!out) return opo::fail(OPO_ERR_ARG, "opo_crystal_properties: null output");
+    if (crystal == OPO_CRYSTAL_ZGP) {  // ZGP.cuh:49-77
+        if (!pol) return opo::fail(OPO_ERR_ARG, "ZGP needs the polarisations of p, s, i");
+        for (int f = 0; f < 3; ++f)
+            if (pol[f] != 'o' && pol[f] != 'e')
+                return opo::fail(OPO_ERR_ARG, "ZGP polarisation %d is '%c', not 'o' or 'e'", f, pol[f]);
+        const double L[3] = {lp, ls, li};
+        const double al[3] = {1.57e-4, 0.17e-6, 0.17e-6};
+        for (int f = 0; f < 3; ++f) {
+            const Zelmon z = zelmon(pol[f]);
+            out->n[f] = z.n(L[f]);
+            out->vg[f] = z.gv(L[f]);
+            out->b2[f] = z.gvd(L[f]);
+            out->b3[f] = 0.0;  // the reference's ZGP has no TOD
+            out->alpha[f] = al[f];
+        }
+        out->d33 = 75.0e-6;
+        out->dq = 2.0 * out->d33 / kPi;
+        out->kp = 2.0 * kPi * out->n[1] / lp;  // ZGP.cuh:71 (ns, as R6)
+        out->ks = 2.0 * kPi * out->n[1] / ls;
+        out->ki = 2.0 * kPi * out->n[2] / ls;  // ZGP.cuh:73 (ls)
+        out->dk = out->kp - out->ks - out->ki;  // no QPM term (QPM = false, :58)
+        return OPO_OK;
+    }
     Sellmeier s;
     double d33, al[3];
     if (crystal == OPO_CRYSTAL_MGOPPLN) {  // MgOPPLN.cuh:59-60, 99-113
@@ -175,8 +231,8 @@ int opo_setup_params(const opo_grid *g, const opo_physical *ph, opo_params *p) {
     if (g->nx < 1 || g->ny < 1 || g->nt < 1 || ph->nz < 2)
         return opo::fail(OPO_ERR_ARG, "opo_setup_params: bad grid or NZ");
     opo_crystal_props cr;
-    int rc = opo_crystal_properties(ph->crystal, ph->temperature, ph->grating_period, ph->lp,
-                                    ph->ls, ph->li, &cr);
+    int rc = opo_crystal_properties_pol(ph->crystal, ph->temperature, ph->grating_period, ph->lp,
+                                        ph->ls, ph->li, ph->pol, &cr);
     if (rc) return rc;
     std::memset(p, 0, sizeof(*p));
     p->lx = ph->lx;
@@ -241,8 +297,8 @@ int opo_setup_pump(const opo_grid *g, const opo_params *p, const opo_physical *p
                    void *out) {
     if (!g || !p || !ph || !out) return opo::fail(OPO_ERR_ARG, "opo_setup_pump: null argument");
     opo_crystal_props cr;
-    int rc = opo_crystal_properties(ph->crystal, ph->temperature, ph->grating_period, ph->lp,
-                                    ph->ls, ph->li, &cr);
+    int rc = opo_crystal_properties_pol(ph->crystal, ph->temperature, ph->grating_period, ph->lp,
+                                        ph->ls, ph->li, ph->pol, &cr);
     if (rc) return rc;
     const int nx = g->nx, ny = g->ny, nt = g->nt;
     const double np = cr.n[0];

@@ -140,3 +140,36 @@ def poisson_requests(n, rate_per_s, seed=5, grids=configs.SERVING_GRIDS, trips=(
         g, p, ph, f = configs.sweep_solve(d, n_round_trips=n_trips, precision=precision)
         out.append(Request(arrival=t, grid=g, params=p, fields=f, n_trips=n_trips, tag=k))
     return out
+
+
+def main(argv=None):
+    """Serving load over several Poisson arrival rates; one JSON line per rate.
+
+    python -m cuopo3d_amd.serving --rates 2,5,10,20,50 --requests 16
+    """
+    import argparse
+    import json
+    from . import autotune_grids
+    ap = argparse.ArgumentParser(description="continuous-batching serving load")
+    ap.add_argument("--rates", default="2,5,10,20,50", help="mean arrivals per second")
+    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--budget-gb", type=float, default=128.0)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--no-tune", action="store_true")
+    args = ap.parse_args(argv)
+    if not args.no_tune:
+        autotune_grids([(nxy, nxy, nt) for nxy, nt in configs.SERVING_GRIDS], reps=3)
+    for rate in [float(x) for x in args.rates.split(",")]:
+        reqs = poisson_requests(args.requests, rate_per_s=rate, seed=args.seed)
+        rep = serve(reqs, budget_bytes=args.budget_gb * (1 << 30), keep_fields=False)
+        d = rep.as_dict()
+        d["arrival_rate_per_s"] = rate
+        d["last_arrival_s"] = max(r.arrival for r in reqs)
+        d["errors"] = sum(1 for r in reqs if r.error)
+        print(json.dumps(d), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

@@ -21,6 +21,9 @@ extern "C" {
 #define OPO_CRYSTAL_MGOPPLN  0  /* MgO:PPLN, Gayer et al.   MgOPPLN.cuh  */
 #define OPO_CRYSTAL_PPLN     1  /* congruent PPLN           PPLN.cuh     */
 #define OPO_CRYSTAL_MGOSPPLT 2  /* MgO:sPPLT, Bruner et al. MgOsPPLT.cuh */
+#define OPO_CRYSTAL_ZGP      3  /* ZnGeP2, Zelmon et al.    ZGP.cuh: birefringent
+                                   phase matching (no grating), polarisations
+                                   of p, s, i chosen by opo_physical.pol    */
 
 /* Pump field modes (EFields3D.cuh:259-293) plus a sech^2 pulse. */
 #define OPO_PUMP_WAVEPLANE_CW     0  /* setWavePlaneCW     EFields3D.cuh:72-85   */
@@ -48,6 +51,10 @@ typedef struct {
     int    n_round_trips;   /* NRT                                           */
     double t_window;        /* > 0 overrides the window trt (ps)             */
     int    precision;       /* OPO_FP32 | OPO_FP64                           */
+    char   pol[4];          /* OPO_CRYSTAL_ZGP: polarisations of p, s, i as
+                               'o'/'e' (ZGP.cuh constructor tuple, e.g.
+                               "eoo"); ignored by the periodically poled
+                               crystals                                      */
 } opo_physical;
 
 /* Crystal properties at (T, Lambda) for the three wavelengths. */
@@ -66,6 +73,12 @@ int opo_slice_count(double lcr, int nz);
  * PPLN.cuh, MgOsPPLT.cuh) in FP64. */
 int opo_crystal_properties(int crystal, double temperature, double grating_period, double lp,
                            double ls, double li, opo_crystal_props *out);
+/* The same with polarisations (three 'o'/'e' characters), required for
+ * OPO_CRYSTAL_ZGP (ZGP.cuh:49-77): n, group velocity and GVD per
+ * polarisation, TOD not modelled (0), kp = 2 pi ns/lp, ks = 2 pi ns/ls,
+ * ki = 2 pi ni/ls and dk = kp - ks - ki with no grating term (:71-74). */
+int opo_crystal_properties_pol(int crystal, double temperature, double grating_period, double lp,
+                               double ls, double li, const char *pol, opo_crystal_props *out);
 
 /* Fill every field of an opo_params from a physical description,
  * following EFields3D.cuh:214-226, Cavity3D.cuh:44-47, Solver3D.cuh:163-165,

@@ -185,3 +185,62 @@ def test_courant_gate():
         g = Grid(n, n, 128)
         p = configs.make_params(g, configs.physical())
         assert LIB.opo_courant_ok(C.byref(p)) == ok, n
+
+
+def zgp(pol, lp=2.09, ls=4.18, li=4.18):
+    cr = Crystal()
+    check(LIB.opo_crystal_properties_pol(configs.CRYSTAL_ZGP, 27.0, 0.0, lp, ls, li, pol, C.byref(cr)),
+          "zgp")
+    return cr
+
+
+@pytest.mark.parametrize("pol", [b"ooo", b"eee"])
+@pytest.mark.parametrize("lam", [2.09, 3.5, 8.0])
+def test_zgp_dispersion_from_finite_differences(pol, lam):
+    """ZGP (ZGP.cuh:101-256): group velocity and GVD against finite
+    differences of its own Sellmeier index, for both polarisations."""
+    n = lambda L: zgp(pol, lp=L).n[0]
+    h1, h2 = 1e-4, 5e-4
+    d1 = (n(lam + h1) - n(lam - h1)) / (2 * h1)
+    d2 = (n(lam + h2) - 2 * n(lam) + n(lam - h2)) / h2 ** 2
+    cr = zgp(pol, lp=lam)
+    assert abs(cr.vg[0] - CLIGHT / (n(lam) - lam * d1)) / cr.vg[0] < 1e-7
+    b2 = lam ** 3 * d2 / (2 * math.pi * CLIGHT ** 2)
+    assert abs(cr.b2[0] - b2) / abs(b2) < 1e-4
+    assert cr.b3[0] == 0.0
+
+
+def test_zgp_index_closed_form_and_reference_quirks():
+    # the Zelmon expression itself at 2.09 um, 'e' and 'o'
+    for pol, (A, B, Cc, D, E) in ((b"eoo", (8.0929, 1.8649, 0.41468, 0.84052, 452.05)),
+                                  (b"oee", (8.0409, 1.68625, 0.40824, 1.2880, 611.05))):
+        L2 = 2.09 ** 2
+        n_ref = math.sqrt(A + B * L2 / (L2 - Cc) + D * L2 / (L2 - E))
+        cr = zgp(pol)
+        assert abs(cr.n[0] - n_ref) < 1e-14
+        assert 3.1 < cr.n[0] < 3.2                       # ZnGeP2 near 2 um
+    cr = zgp(b"eoo")
+    assert cr.n[0] > zgp(b"ooo").n[0]                    # positive uniaxial
+    # ZGP.cuh:71-74: kp uses ns, ki uses ls, no grating term (QPM = false)
+    assert abs(cr.kp - 2 * math.pi * cr.n[1] / 2.09) < 1e-12
+    assert abs(cr.ki - 2 * math.pi * cr.n[2] / 4.18) < 1e-12
+    assert abs(cr.dk - (cr.kp - cr.ks - cr.ki)) < 1e-12
+    assert abs(cr.dq - 2 * 75e-6 / math.pi) < 1e-18
+    assert list(cr.alpha) == [1.57e-4, 0.17e-6, 0.17e-6]
+
+
+def test_zgp_requires_polarisations():
+    cr = Crystal()
+    assert LIB.opo_crystal_properties(configs.CRYSTAL_ZGP, 27.0, 0.0, 2.09, 4.18, 4.18, C.byref(cr)) != 0
+    assert LIB.opo_crystal_properties_pol(configs.CRYSTAL_ZGP, 27.0, 0.0, 2.09, 4.18, 4.18, b"exo",
+                                          C.byref(cr)) != 0
+    # a full solve set-up with ZGP: kappa from dQ = 2 d33/pi and n per polarisation
+    ph = configs.physical(lcr=5000.0, dq=0.0)
+    ph.crystal = configs.CRYSTAL_ZGP
+    ph.lp, ph.ls, ph.li = 2.09, 4.18, 4.18
+    ph.pol = b"eoo"
+    p = configs.make_params(Grid(32, 32, 64), ph)
+    cr = zgp(b"eoo")
+    for f, lam in enumerate((2.09, 4.18, 4.18)):
+        assert abs(p.kappa[f] - 2 * math.pi * cr.dq / (cr.n[f] * lam)) < 1e-15
+    assert p.kdiff[0] == cr.kp and p.dk == cr.dk
