@@ -46,6 +46,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
     ap.add_argument("--tune-reps", type=int, default=5, help="timed stretches per tuning candidate")
+    ap.add_argument("--horizon-trips", type=int, default=0,
+                    help="also compare FP32 and FP64 device fields with the FP64 oracle over this "
+                         "many round trips (checkpoints 1, 3, 10, 30, 100, 200, ...); off by default")
     ap.add_argument("--grid", type=int, nargs=2, default=(64, 256), metavar=("NXY", "NT"),
                     help="grid of the cavity workload (default 64 256); other grids are for "
                          "profiling and skip the CPU and comparator legs")
@@ -133,6 +136,7 @@ def main():
 
     if default_grid:
         out["idler_dumped"] = idler_dumped_leg(co, configs, args.steps)
+        out["fp64_path"] = fp64_leg(co, configs, min(args.steps, 50))
     if not args.no_cpu:
         from oracle.pyoracle import Oracle, get_threads
         g2, p2, _, f2 = configs.cavity_64(n_round_trips=args.cpu_trips)
@@ -160,11 +164,68 @@ def main():
         out["plain_rocfft"] = plain_leg()
         if "value" in out["plain_rocfft"]:
             out["plain_rocfft"]["fused_speedup"] = value / out["plain_rocfft"]["value"]
+    if args.horizon_trips > 0:
+        out["horizon"] = horizon_leg(co, configs, args.horizon_trips)
     if args.sweep_trips > 0:
         out["sweep"] = sweep_leg(co, configs, args.sweep_trips)
     if args.serve_requests > 0:
         out["serving"] = serving_leg(args.serve_requests, args.serve_rate)
     print(json.dumps(out))
+
+
+def horizon_leg(co, configs, trips):
+    """Error growth with the number of round trips: FP32 and FP64 device
+    fields and per-trip powers against the FP64 oracle at checkpoints."""
+    from oracle.pyoracle import Oracle
+    from cuopo3d_amd.capi import OPO_FP64
+    g, p32, _, f32 = configs.cavity_64(n_round_trips=trips)
+    _, p64, _, _ = configs.cavity_64(n_round_trips=trips, precision=OPO_FP64)
+    f64 = tuple(x.astype(np.complex128) for x in f32)
+    o = Oracle(g, p64)
+    o.set_fields(*f64)
+    s32 = co.Solver(g, p32, device=0)
+    s32.set_fields(*f32)
+    s64 = co.Solver(g, p64, device=0)
+    s64.set_fields(*f64)
+    marks = [m for m in (1, 3, 10, 30, 100, 200, 500, 1000) if m < trips] + [trips]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    res = {"checkpoints": marks, "fp32_field_rel_l2_p_s_i": [], "fp64_field_rel_l2_p_s_i": [],
+           "fp32_signal_power_rel_max": [], "fp64_signal_power_rel_max": []}
+    done, pw_o, pw_32, pw_64 = 0, [], [], []
+    for m in marks:
+        n = m - done
+        pw_o.append(o.run_round_trips(n))
+        pw_32.append(s32.runOPO(n))
+        pw_64.append(s64.runOPO(n))
+        done = m
+        r = o.get_fields()
+        res["fp32_field_rel_l2_p_s_i"].append([rel(a, b) for a, b in zip(s32.get_fields(), r)])
+        res["fp64_field_rel_l2_p_s_i"].append([rel(a, b) for a, b in zip(s64.get_fields(), r)])
+        po = np.concatenate(pw_o)[:, 1]
+        res["fp32_signal_power_rel_max"].append(float(np.max(np.abs(np.concatenate(pw_32)[:, 1] - po) / po)))
+        res["fp64_signal_power_rel_max"].append(float(np.max(np.abs(np.concatenate(pw_64)[:, 1] - po) / po)))
+        print(f"horizon trip {m}: fp32 {res['fp32_field_rel_l2_p_s_i'][-1]}", file=sys.stderr, flush=True)
+    s32.close()
+    s64.close()
+    return res
+
+
+def fp64_leg(co, configs, trips):
+    """The same cavity on the all-FP64 device path (the long-horizon mode)."""
+    from cuopo3d_amd.capi import OPO_FP64
+    g, p, _, f = configs.cavity_64(n_round_trips=trips + 1, precision=OPO_FP64)
+    s = co.Solver(g, p, device=0)
+    s.set_fields(*f)
+    s.runOPO(1)
+    s.sync()
+    t0 = time.perf_counter()
+    s.runOPO(trips)
+    s.sync()
+    dt = time.perf_counter() - t0
+    s.close()
+    cells = g.nx * g.ny * g.nt
+    return {"trips": trips, "value": cells * p.n_slices * trips / dt, "unit": "cell-steps/s",
+            "ms_per_step": dt / trips * 1e3}
 
 
 def idler_dumped_leg(co, configs, trips):

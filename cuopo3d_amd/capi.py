@@ -58,6 +58,10 @@ class Params(C.Structure):
         ("n_round_trips", C.c_int),
         ("power_prefactor", C.c_double * 3),
         ("precision", C.c_int),
+        ("eom_on", C.c_int),
+        ("eom_beta", C.c_double),
+        ("eom_fpm", C.c_double),
+        ("no_diffraction", C.c_int),
     ]
 
     def copy(self):

@@ -32,6 +32,7 @@ __device__ __forceinline__ Rk4Args<T> batch_rk(const SolveDesc<T> &d, int mode) 
     r.mi_im = d.mi_im;
     r.mi_lo_re = d.mi_lo_re;
     r.mi_lo_im = d.mi_lo_im;
+    r.eom = mode == BATCH_LAST ? d.eom : nullptr;
     return r;
 }
 

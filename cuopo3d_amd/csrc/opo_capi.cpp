@@ -140,7 +140,7 @@ template <typename T> struct SolveT final : SolveImpl {
     C *Api = nullptr, *A[3] = {nullptr, nullptr, nullptr};
     C *tables = nullptr;
     C *tw_x = nullptr, *tw_t = nullptr, *tabT[3] = {}, *tabG = nullptr, *qx[3] = {}, *qy[3] = {},
-      *qfsx[2] = {}, *qfsy[2] = {};
+      *qfsx[2] = {}, *qfsy[2] = {}, *tabE = nullptr;
     double *dbuf = nullptr;   // partials | pw3
     double *partial = nullptr, *pw3 = nullptr;
     double *series = nullptr;
@@ -196,13 +196,13 @@ template <typename T> struct SolveT final : SolveImpl {
         const int nx = g.nx, ny = g.ny, nt = g.nt;
         // Every entry is stored as a (hi, lo) pair: hi = value rounded to T,
         // lo = rounding residual (zero for T = double).
-        const size_t total = 2 * ((size_t)nx + nt + 3 * nt + nt + 3 * nx + 3 * ny + 2 * nx + 2 * ny);
+        const size_t total = 2 * ((size_t)nx + nt + 3 * nt + nt + 3 * nx + 3 * ny + 2 * nx + 2 * ny + nt);
         std::vector<C> h(total);
         size_t off = 0;
         auto take = [&](size_t n) { size_t o = off; off += 2 * n; return o; };
         const size_t o_twx = take(nx), o_twt = take(nt), o_T = take(3 * (size_t)nt), o_G = take(nt),
                      o_qx = take(3 * (size_t)nx), o_qy = take(3 * (size_t)ny),
-                     o_fx = take(2 * (size_t)nx), o_fy = take(2 * (size_t)ny);
+                     o_fx = take(2 * (size_t)nx), o_fy = take(2 * (size_t)ny), o_E = take(nt);
         // put(i, re, im): the pair of one entry at h[i] (hi) and h[i + 1] (lo);
         // entry j of a table starting at o lives at i = o + 2 j.
         auto put = [&](size_t i, double re, double im) {
@@ -269,6 +269,17 @@ template <typename T> struct SolveT final : SolveImpl {
                 put(o_fy + 2 * ((size_t)f * ny + j), c / ny, s / ny);
             }
         }
+        // Modulator on s and i: exp(i beta sin(2 pi fpm t_j)) on the time grid
+        // t_j = -W/2 + j W/(NT-1) (KernelEOM, Solver1D.cuh:62-72).
+        if (p.eom_on) {
+            std::vector<double> t(nt);
+            opo_time_grid(nt, p.t_window, t.data());
+            for (int k = 0; k < nt; ++k) {
+                double c, s2;
+                cis(p.eom_beta * std::sin(kTwoPi * std::fmod(p.eom_fpm * t[k], 1.0)), c, s2);
+                put(o_E + 2 * (size_t)k, c, s2);
+            }
+        }
         if (hipMalloc(&tables, total * sizeof(C)) != hipSuccess)
             return fail(OPO_ERR_NOMEM, "hipMalloc(tables) failed");
         CK(hipMemcpy(tables, h.data(), total * sizeof(C), hipMemcpyHostToDevice));
@@ -280,6 +291,7 @@ template <typename T> struct SolveT final : SolveImpl {
             qy[f] = tables + o_qy + 2 * (size_t)f * ny;
         }
         tabG = tables + o_G;
+        tabE = p.eom_on ? tables + o_E : nullptr;
         for (int f = 0; f < 2; ++f) {
             qfsx[f] = tables + o_fx + 2 * (size_t)f * nx;
             qfsy[f] = tables + o_fy + 2 * (size_t)f * ny;
@@ -319,6 +331,7 @@ template <typename T> struct SolveT final : SolveImpl {
         a.ms_lo_im = (T)(p.fb_im[0] - (double)a.ms_im);
         a.mi_lo_re = (T)(p.fb_re[1] - (double)a.mi_re);
         a.mi_lo_im = (T)(p.fb_im[1] - (double)a.mi_im);
+        a.eom = mirror ? tabE : nullptr;
         return a;
     }
 
@@ -352,7 +365,7 @@ template <typename T> struct SolveT final : SolveImpl {
     } while (0)
 
     int q_rk4(const C *psrc, const Rk4Args<T> &a, const int *st) {
-        PROF(OPO_STAGE_NONLINEAR, (launch_rk4<T>(stream, psrc, A[0], A[1], A[2], N, a, st)));
+        PROF(OPO_STAGE_NONLINEAR, (launch_rk4<T>(stream, psrc, A[0], A[1], A[2], N, P, a, st)));
         return OPO_OK;
     }
     int q_dispersion(const int *st) {
@@ -418,7 +431,7 @@ template <typename T> struct SolveT final : SolveImpl {
             Rk4Args<T> rk = k == 0 ? rk_args(1, z, 0.0, false) : rk_args(2, (k - 1) * p.dz, z, false);
             if ((rc = q_nl_time(psrc, rk, tabT[0], (int)(tabT[1] - tabT[0]), 0, OPO_STAGE_TIME_OP, st)))
                 return rc;
-            if ((rc = q_diffraction(st))) return rc;
+            if (!p.no_diffraction && (rc = q_diffraction(st))) return rc;
         }
         const Rk4Args<T> last = rk_args(1, (S - 1) * p.dz, 0.0, trip);
         if (trip && p.gdd_on) return q_nl_time(A[0], last, tabG, 0, 1, OPO_STAGE_GDD, st);
@@ -429,7 +442,7 @@ template <typename T> struct SolveT final : SolveImpl {
     int q_trip(bool fs) {
         int rc = q_ssfm(true, flags);  // ends with the mirror and GDD
         if (rc) return rc;
-        if (fs && (rc = q_free_space(flags))) return rc;
+        if (fs && !p.no_diffraction && (rc = q_free_space(flags))) return rc;
         int prc = prof_mark(OPO_STAGE_POWER);
         if (prc) return prc;
         CK(launch_power_partial<T>(stream, A[0], A[1], A[2], N, nblk, partial, flags));
@@ -695,6 +708,7 @@ template <typename T> struct SolveT final : SolveImpl {
         d.ny = g.ny;
         d.nt = g.nt;
         d.gdd_on = p.gdd_on;
+        d.eom = tabE;
         const Rk4Args<T> r = rk_args(1, 0.0, 0.0, true);
         for (int f = 0; f < 3; ++f) {
             d.kc1[f] = r.kc1[f];
@@ -743,6 +757,8 @@ static int validate(const opo_grid *g, const opo_params *p) {
         if (!(p->kdiff[f] > 0)) return fail(OPO_ERR_ARG, "kdiff must be > 0");
     if ((g->nx > 1 && !(p->dx > 0)) || (g->ny > 1 && !(p->dy > 0)))
         return fail(OPO_ERR_ARG, "dx, dy must be > 0");
+    if (p->eom_on && g->nt > 1 && !(p->t_window > 0))
+        return fail(OPO_ERR_ARG, "the modulator needs t_window > 0 (its time grid)");
     return OPO_OK;
 }
 
@@ -1109,6 +1125,8 @@ int opo_batch_submit(opo_batch *b, const opo_grid *g, const opo_params *p, const
         return fail(OPO_ERR_UNSUPPORTED, "grid %dx%dx%d is not supported in a batch", g->nx, g->ny, g->nt);
     if (p->dk_phase_scale != 0.0 && p->dk != 0.0)
         return fail(OPO_ERR_UNSUPPORTED, "a batch runs solves without the dk phase term");
+    if (p->no_diffraction)
+        return fail(OPO_ERR_UNSUPPORTED, "a batch runs solves with diffraction (no plane-wave model)");
     const size_t fp = opo_batch_footprint(g, p->precision);
     if (fp > b->budget) return fail(OPO_ERR_BUDGET, "solve needs %zu bytes, budget is %zu", fp, b->budget);
     std::lock_guard<std::mutex> lk(b->mu);

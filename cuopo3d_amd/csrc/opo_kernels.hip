@@ -22,7 +22,7 @@ namespace opo {
 
 template <typename T, bool PHASE>
 __global__ void __launch_bounds__(256) k_rk4(const cplx<T> *__restrict__ psrc, cplx<T> *ap,
-                                             cplx<T> *as, cplx<T> *ai, size_t n, Rk4Args<T> a,
+                                             cplx<T> *as, cplx<T> *ai, size_t n, size_t plane, Rk4Args<T> a,
                                              const int *status) {
     if (status && *status) return;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -31,6 +31,11 @@ __global__ void __launch_bounds__(256) k_rk4(const cplx<T> *__restrict__ psrc, c
         rk4_half_step<T, PHASE>(a, 0, A);
         if (a.nsteps > 1) rk4_half_step<T, PHASE>(a, 1, A);
         if (a.out_scale) {
+            if (a.eom) {  // KernelEOM (Solver1D.cuh:62-72) at time sample i / plane
+                const size_t t = i / plane;
+                A.s = cmul_hl(A.s, a.eom[2 * t], a.eom[2 * t + 1]);
+                A.i = cmul_hl(A.i, a.eom[2 * t], a.eom[2 * t + 1]);
+            }
             // roundTripChange (Cavity3D.cuh:12-28): (sqrt(R) e^{i phi}) * A
             A.s = cmul_hl(A.s, cplx<T>{a.ms_re, a.ms_im}, cplx<T>{a.ms_lo_re, a.ms_lo_im});
             A.i = cmul_hl(A.i, cplx<T>{a.mi_re, a.mi_im}, cplx<T>{a.mi_lo_re, a.mi_lo_im});
@@ -43,15 +48,15 @@ __global__ void __launch_bounds__(256) k_rk4(const cplx<T> *__restrict__ psrc, c
 
 template <typename T>
 hipError_t launch_rk4(hipStream_t st, const cplx<T> *psrc, cplx<T> *ap, cplx<T> *as, cplx<T> *ai,
-                      size_t n, const Rk4Args<T> &a, const int *status) {
+                      size_t n, size_t plane, const Rk4Args<T> &a, const int *status) {
     const int bs = 256;
     size_t want = (n + bs - 1) / bs;
     int grid = (int)(want < 8192 ? want : 8192);
     if (grid < 1) grid = 1;
     if (a.has_phase)
-        hipLaunchKernelGGL((k_rk4<T, true>), dim3(grid), dim3(bs), 0, st, psrc, ap, as, ai, n, a, status);
+        hipLaunchKernelGGL((k_rk4<T, true>), dim3(grid), dim3(bs), 0, st, psrc, ap, as, ai, n, plane, a, status);
     else
-        hipLaunchKernelGGL((k_rk4<T, false>), dim3(grid), dim3(bs), 0, st, psrc, ap, as, ai, n, a, status);
+        hipLaunchKernelGGL((k_rk4<T, false>), dim3(grid), dim3(bs), 0, st, psrc, ap, as, ai, n, plane, a, status);
     return hipGetLastError();
 }
 
@@ -499,7 +504,7 @@ hipError_t launch_power_final(hipStream_t st, const double *partial, int nblk, d
 // ------------------------------------------------------- instantiation
 #define OPO_INST(T)                                                                               \
     template hipError_t launch_rk4<T>(hipStream_t, const cplx<T> *, cplx<T> *, cplx<T> *,          \
-                                      cplx<T> *, size_t, const Rk4Args<T> &, const int *);        \
+                                      cplx<T> *, size_t, size_t, const Rk4Args<T> &, const int *); \
     template hipError_t launch_tline<T>(hipStream_t, const Geometry &, const FieldOps<T> &, int,   \
                                         const cplx<T> *, const int *);                            \
     template hipError_t launch_tline3<T>(hipStream_t, const Geometry &, const TLine3Args<T> &,      \

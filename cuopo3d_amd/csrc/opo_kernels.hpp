@@ -21,6 +21,10 @@ template <typename T> struct Rk4Args {
     int out_scale;     // multiply s and i by the mirror factors after the step
     T ms_re, ms_im, mi_re, mi_im;          // mirror factors (rounded)
     T ms_lo_re, ms_lo_im, mi_lo_re, mi_lo_im;  // their rounding residuals
+    // With out_scale: the modulator factor of time sample t as a (hi, lo)
+    // pair at eom[2t], eom[2t+1], applied to s and i before the mirror;
+    // nullptr when there is no modulator.
+    const cplx<T> *eom;
 };
 
 // A set of up to 3 fields, each with its own diagonal table.
@@ -55,7 +59,7 @@ hipError_t launch_tline3(hipStream_t st, const Geometry &g, const TLine3Args<T> 
 // Launchers; return hipError_t of the launch.
 template <typename T>
 hipError_t launch_rk4(hipStream_t st, const cplx<T> *psrc, cplx<T> *ap, cplx<T> *as, cplx<T> *ai,
-                      size_t n, const Rk4Args<T> &a, const int *status);
+                      size_t n, size_t plane, const Rk4Args<T> &a, const int *status);
 
 // Along-t operator F_t^{-}( tab . F_t^{+}(A) ) for nf fields (blockIdx.y = field).
 template <typename T>
@@ -110,6 +114,7 @@ template <typename T> struct SolveDesc {
     const cplx<T> *qx, *qy;        // Q factor tables (3 fields, stride 2nx / 2ny)
     const cplx<T> *qfsx, *qfsy;    // free-space factor tables (s, i)
     int nx, ny, nt, gdd_on;
+    const cplx<T> *eom;            // modulator table (hi/lo pairs), nullptr when off
     T kc1[3], kc3[3], kh6[3];      // nonlinear step weights (see Rk4Args)
     T ms_re, ms_im, ms_lo_re, ms_lo_im, mi_re, mi_im, mi_lo_re, mi_lo_im;  // mirror
     double pref[3];                // power prefactors

@@ -89,7 +89,7 @@ template <typename T> struct alignas(4 * sizeof(T)) cpair {
 // Pointwise part of the fused pass for two cells: RK4 half step(s), then
 // the mirror factors on s and i when the pass closes a round trip.
 template <typename T, bool PHASE>
-__device__ __forceinline__ void nl_point_pair(const Rk4Args<T> &rk, Tri<T> &A0, Tri<T> &A1) {
+__device__ __forceinline__ void nl_point_pair(const Rk4Args<T> &rk, int t, Tri<T> &A0, Tri<T> &A1) {
     if (rk.nsteps > 0) {
         rk4_half_step<T, PHASE>(rk, 0, A0);
         rk4_half_step<T, PHASE>(rk, 0, A1);
@@ -99,6 +99,13 @@ __device__ __forceinline__ void nl_point_pair(const Rk4Args<T> &rk, Tri<T> &A0, 
         }
     }
     if (rk.out_scale) {
+        if (rk.eom) {
+            const cplx<T> e = rk.eom[2 * t], el = rk.eom[2 * t + 1];
+            A0.s = cmul_hl(A0.s, e, el);
+            A1.s = cmul_hl(A1.s, e, el);
+            A0.i = cmul_hl(A0.i, e, el);
+            A1.i = cmul_hl(A1.i, e, el);
+        }
         const cplx<T> ms = {rk.ms_re, rk.ms_im}, msl = {rk.ms_lo_re, rk.ms_lo_im};
         const cplx<T> mi = {rk.mi_re, rk.mi_im}, mil = {rk.mi_lo_re, rk.mi_lo_im};
         A0.s = cmul_hl(A0.s, ms, msl);
@@ -121,7 +128,9 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     const size_t plane = (size_t)nx * ny;
     const size_t base = (size_t)y * nx + x0;
     const Rk4Args<T> &rk = a.rk;
-    // pointwise step in registers, two pixels per thread per iteration
+    // pointwise step in registers, two pixels per thread per iteration (a
+    // version that issued the loads of four iterations ahead of the
+    // arithmetic was measured slower: more registers, no latency gained)
     if constexpr (TX % 2 == 0) {
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const int t = i / TX, xl = i % TX;
@@ -130,7 +139,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             cpair<T> s = *reinterpret_cast<const cpair<T> *>(a.f[1] + gi);
             cpair<T> q = *reinterpret_cast<const cpair<T> *>(a.f[2] + gi);
             Tri<T> A0 = {p.a, s.a, q.a}, A1 = {p.b, s.b, q.b};
-            nl_point_pair<T, PHASE>(rk, A0, A1);
+            nl_point_pair<T, PHASE>(rk, t, A0, A1);
             const int li = L::at(i);
             lds[li] = A0.p;
             lds[li + 1] = A1.p;
@@ -147,6 +156,11 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             if (rk.nsteps > 0) rk4_half_step<T, PHASE>(rk, 0, A);
             if (rk.nsteps > 1) rk4_half_step<T, PHASE>(rk, 1, A);
             if (rk.out_scale) {
+                if (rk.eom) {
+                    const cplx<T> e = rk.eom[2 * t], el = rk.eom[2 * t + 1];
+                    A.s = cmul_hl(A.s, e, el);
+                    A.i = cmul_hl(A.i, e, el);
+                }
                 A.s = cmul_hl(A.s, cplx<T>{rk.ms_re, rk.ms_im}, cplx<T>{rk.ms_lo_re, rk.ms_lo_im});
                 A.i = cmul_hl(A.i, cplx<T>{rk.mi_re, rk.mi_im}, cplx<T>{rk.mi_lo_re, rk.mi_lo_im});
             }
@@ -243,7 +257,7 @@ __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T
             const int i = 2 * tid + j * 2 * BS;
             if (EVEN || i < TILE) {
                 Tri<T> A0 = {rp[j].a, rs[j].a, ri[j].a}, A1 = {rp[j].b, rs[j].b, ri[j].b};
-                nl_point_pair<T, false>(a.rk, A0, A1);
+                nl_point_pair<T, false>(a.rk, i / TX, A0, A1);
                 const int li = L::at(i);
                 lds[li] = A0.p;
                 lds[li + 1] = A1.p;

@@ -112,6 +112,17 @@ typedef struct {
                              0.5*EPS0*C*ns*dx*dy/(LX*LY)  Solver3D.cuh:542  */
 
     int    precision;     /* OPO_FP32 or OPO_FP64                           */
+
+    /* intracavity elements of the plane-wave model (Solver1D.cuh) ------- */
+    int    eom_on;        /* electro-optic modulator on s and i once per
+                             round trip, after the crystal and before GDD:
+                             A(t) *= exp(i eom_beta sin(2 pi eom_fpm t))
+                             (KernelEOM, Solver1D.cuh:62-72, 467)           */
+    double eom_beta;      /* modulation depth (rad)                         */
+    double eom_fpm;       /* modulation frequency (THz = 1/ps)              */
+    int    no_diffraction;/* 1: the plane-wave model, no transverse step in
+                             the crystal or in free space (Solver1D.cuh
+                             SSFM :436-445); use with NX = NY = 1           */
 } opo_params;
 
 typedef struct opo_solve opo_solve;

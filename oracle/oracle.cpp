@@ -198,6 +198,7 @@ struct Base {
     virtual void diffraction() = 0;
     virtual void free_space() = 0;
     virtual void gdd() = 0;
+    virtual void eom() = 0;
     virtual void mirror() = 0;
     virtual void power(double out[3]) = 0;
     virtual void table(int which, double *out) = 0;
@@ -461,6 +462,20 @@ template <typename R> struct Solve final : Base {
         t_operator(A[1], egdd);
         t_operator(A[2], egdd);
     }
+    // KernelEOM (Solver1D.cuh:62-72) on As and Ai: A(t) *= exp(i beta
+    // sin(2 pi fpm t)), t = linspace(-W/2, W/2, NT) (Common.cuh:108-118).
+    void eom() override {
+        const int nt = g.nt;
+        for (int k = 0; k < nt; ++k) {
+            const double t = nt > 1 ? -0.5 * p.t_window + k * p.t_window / (nt - 1) : 0.0;
+            const double ph = p.eom_beta * std::sin(2.0 * M_PI * p.eom_fpm * t);
+            const cx<R> m = {(R)std::cos(ph), (R)std::sin(ph)};
+            for (int f = 1; f < 3; ++f) {
+                cx<R> *a = A[f].data() + (size_t)k * P;
+                for (size_t i = 0; i < P; ++i) a[i] = m * a[i];
+            }
+        }
+    }
     // roundTripChange (Cavity3D.cuh:12-28): A <- (sqrt(R) e^{i phi}) A for
     // the s and i multipliers (1 + 0i for a non-resonant field, R13).
     void mirror() override {
@@ -542,6 +557,7 @@ void oracle_step_diffraction(oracle_solve *o) { o->s->diffraction(); }
 void oracle_step_free_space(oracle_solve *o) { o->s->free_space(); }
 void oracle_step_gdd(oracle_solve *o) { o->s->gdd(); }
 void oracle_step_mirror(oracle_solve *o) { o->s->mirror(); }
+void oracle_step_eom(oracle_solve *o) { o->s->eom(); }
 
 // SSFM (Solver3D.cuh:946-960): per slice N(dz/2) D(dz) Q(dz) N(dz/2), with
 // the slice position z_k = k*dz passed to both half steps.
@@ -551,7 +567,7 @@ void oracle_ssfm(oracle_solve *o) {
         double z = k * s.p.dz;
         s.nonlinear(z);
         s.dispersion();
-        s.diffraction();
+        if (!s.p.no_diffraction) s.diffraction();  // plane-wave model: Solver1D.cuh:436-445
         s.nonlinear(z);
     }
 }
@@ -565,7 +581,8 @@ int oracle_run_round_trips(oracle_solve *o, int n, double *power) {
         o->reinject();
         oracle_ssfm(o);
         bool last = s.p.skip_fs_on_last_trip && rt == s.p.n_round_trips - 1;
-        if (s.p.fs_on && !last) s.free_space();
+        if (s.p.fs_on && !last && !s.p.no_diffraction) s.free_space();
+        if (s.p.eom_on) s.eom();  // Solver1D.cuh:467, before GDD
         if (s.p.gdd_on) s.gdd();
         s.mirror();
         double pw[3];

@@ -42,6 +42,7 @@ void oracle_step_diffraction(oracle_solve *o); /* Solver3D.cuh:725-788 */
 void oracle_step_free_space(oracle_solve *o);  /* Solver3D.cuh:817-869 */
 void oracle_step_gdd(oracle_solve *o);         /* Solver3D.cuh:643-694 */
 void oracle_step_mirror(oracle_solve *o);      /* Cavity3D.cuh:67-80   */
+void oracle_step_eom(oracle_solve *o);         /* Solver1D.cuh:62-72   */
 void oracle_ssfm(oracle_solve *o);             /* Solver3D.cuh:946-960 */
 /* n round trips (Solver3D.cuh:979-1011); power: n*3 doubles or NULL.
  * Returns the number of trips completed (stops after a non-finite signal

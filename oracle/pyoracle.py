@@ -39,6 +39,7 @@ def load():
         "oracle_step_free_space": (None, [vp]),
         "oracle_step_gdd": (None, [vp]),
         "oracle_step_mirror": (None, [vp]),
+        "oracle_step_eom": (None, [vp]),
         "oracle_ssfm": (None, [vp]),
         "oracle_run_round_trips": (ip, [vp, ip, dp]),
         "oracle_power": (None, [vp, dp]),
@@ -136,6 +137,9 @@ class Oracle:
 
     def mirror(self):
         self.lib.oracle_step_mirror(self.h)
+
+    def eom(self):
+        self.lib.oracle_step_eom(self.h)
 
     def ssfm(self):
         self.lib.oracle_ssfm(self.h)

@@ -151,3 +151,28 @@ def test_sweep_cli_matches_single_solves(tmp_path, require_gpu):
             ref = s.runOPO(3)
             # the file holds 17 significant digits of the batch's doubles
             assert np.allclose(rows[:, 1:], ref, rtol=1e-15, atol=0), (w, pw)
+
+
+def test_batch_with_eom_equals_single_solves(require_gpu):
+    """Solves with the modulator in a batch give the single-solve fields bitwise."""
+    solves = mix(3, trips=2)
+    for (g, p, ph, f) in solves:
+        p.eom_on, p.eom_beta, p.eom_fpm = 1, 0.6, 2.0 / p.t_window
+    mixed, _ = run_and_collect(solves, 2)
+    for (g, p, ph, f), (pb, fb) in zip(solves, mixed):
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        ps = s.runOPO(2)
+        for a, b in zip(s.get_fields(), fb):
+            assert np.array_equal(a, b)
+        assert np.allclose(ps, pb, rtol=1e-13, atol=0)
+
+
+def test_batch_rejects_plane_wave_solves():
+    from cuopo3d_amd.capi import Grid
+    g, p, ph, f = configs.cavity_64(n=(32, 32, 64), n_round_trips=1)
+    p.no_diffraction = 1
+    b = co.Batch(1 << 30)
+    with pytest.raises(RuntimeError, match="plane-wave"):
+        b.submit(g, p, *f, 1)
+    b.close()

@@ -226,3 +226,74 @@ def test_nan_stops_round_trips():
     o.set_fields(a, s, a)
     pw = o.run_round_trips(5)
     assert pw.shape[0] == 1 and np.isnan(pw[0, 1])
+
+
+def test_eom_closed_form():
+    """KernelEOM (Solver1D.cuh:62-72): s and i times exp(i beta sin(2 pi fpm t))
+    on t = linspace(-W/2, W/2, NT); the pump untouched."""
+    nt, nx, W, beta, fpm = 64, 4, 20.0, 0.7, 0.3
+    g, p, _ = simple_params(nt=nt, nx=nx, ny=nx, t_window=W)
+    p.eom_on, p.eom_beta, p.eom_fpm = 1, beta, fpm
+    rng = np.random.default_rng(21)
+    f0 = [rng.standard_normal((nt, nx, nx)) + 1j * rng.standard_normal((nt, nx, nx)) for _ in range(3)]
+    o = Oracle(g, p)
+    o.set_fields(*f0)
+    o.eom()
+    f = o.get_fields()
+    t = np.linspace(-W / 2, W / 2, nt).reshape(nt, 1, 1)
+    m = np.exp(1j * beta * np.sin(2 * np.pi * fpm * t))
+    assert np.array_equal(f[0], f0[0])
+    assert np.abs(f[1] - m * f0[1]).max() < 1e-13
+    assert np.abs(f[2] - m * f0[2]).max() < 1e-13
+
+
+def test_trip_applies_eom_before_gdd():
+    """A trip with no slices is Api -> Ap, then EOM, then GDD (Solver1D.cuh:467-468):
+    the modulated pulse must come out dispersed, which EOM after GDD would not give."""
+    nt, dt, T0, gdd, beta, fpm = 256, 0.05, 0.5, 0.05, 1.3, 0.8
+    W = dt * (nt - 1)
+    t = (np.arange(nt) - (nt - 1) / 2) * dt          # linspace(-W/2, W/2, NT)
+    g, p, _ = simple_params(nt=nt, omega=dft_omega(nt, dt), t_window=W, n_slices=0)
+    p.gdd, p.gdd_on = gdd, 1
+    p.eom_on, p.eom_beta, p.eom_fpm = 1, beta, fpm
+    a0 = np.exp(-t ** 2 / (2 * T0 ** 2)).astype(complex)
+    o = Oracle(g, p)
+    o.set_fields(a0.reshape(nt, 1, 1), a0.reshape(nt, 1, 1), a0.reshape(nt, 1, 1))
+    o.run_round_trips(1)
+    got = o.get_fields()[1].ravel()
+    # closed-form order: modulate in time, then exp(i w^2 GDD / 2) on the spectrum
+    m = a0 * np.exp(1j * beta * np.sin(2 * np.pi * fpm * t))
+    w = dft_omega(nt, dt)
+    want = np.fft.fft(np.fft.ifft(m) * np.exp(0.5j * w * w * gdd))
+    assert np.abs(got - want).max() < 1e-12
+    wrong = np.fft.fft(np.fft.ifft(a0) * np.exp(0.5j * w * w * gdd)) * np.exp(1j * beta * np.sin(2 * np.pi * fpm * t))
+    assert np.abs(got - wrong).max() > 1e-3
+
+
+def test_plane_wave_mode_equals_one_pixel_without_absorption():
+    """no_diffraction (the plane-wave model, Solver1D.cuh:436-445) skips Q; on a
+    1x1 grid without absorption Q is the identity, so both give the same fields."""
+    nt = 64
+    g, p, _ = simple_params(nt=nt, dz=5.0, n_slices=4, beta2=(1e-2, 2e-2, 3e-2),
+                            delta=(0.1, 0.0, -0.1), kappa=(1e-3, 2e-3, 2e-3), t_window=10.0)
+    rng = np.random.default_rng(4)
+    f0 = [(rng.standard_normal(nt) + 1j * rng.standard_normal(nt)).reshape(nt, 1, 1) for _ in range(3)]
+    outs = []
+    for nd in (0, 1):
+        p.no_diffraction = nd
+        o = Oracle(g, p)
+        o.set_fields(*f0)
+        o.ssfm()
+        outs.append(o.get_fields())
+    for a, b in zip(*outs):
+        assert np.abs(a - b).max() < 1e-13
+    # with absorption the transverse step carries exp(-alpha dz/2) (R8), so they differ
+    p.alpha[1] = 1e-3
+    res = []
+    for nd in (0, 1):
+        p.no_diffraction = nd
+        o = Oracle(g, p)
+        o.set_fields(*f0)
+        o.ssfm()
+        res.append(o.get_fields()[1])
+    assert np.abs(res[0] - res[1]).max() > 1e-6
