@@ -72,7 +72,9 @@ constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 6
 // 1024 threads; V >= 3 persistent workgroups (k + 1 per CU) walking a run
 // of planes with the next plane prefetched into registers,
 // V - 3 = 2 b + k with the same block sizes.  Persistent variants need at
-// most 16 prefetched elements per thread.
+// most 16 prefetched elements per thread.  (All three fields of one time
+// sample per workgroup was measured slower than one plane per workgroup
+// on every grid and is not offered.)
 constexpr int kXyVariants = 9;
 constexpr int xy_variant_bs(int v) {
     const int b = v < 3 ? v : (v - 3) / 2;
