@@ -130,6 +130,12 @@ def main():
         "stages": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in stages.items()},
         "gpu_kernel_ms_per_trip": gpu_kernel_ms,
+        # The same rate priced in the 5-pass accounting of a split-step step
+        # (RK4 pair, forward xy, forward t with the propagator, inverse t,
+        # inverse xy: 5 x 3 fields x 16 B = 240 B per cell-step): the HBM
+        # rate an unfused schedule would need to match this throughput.
+        "five_pass_equivalent": {"bytes_per_cell_step": 240, "GBs": round(value * 240 / 1e9, 1),
+                                 "frac_of_peak": round(value * 240 / 1e9 / HBM_PEAK_GBS, 4)},
         "autotune": tune_report.strip().splitlines(),
         "signal_power_last_trip_W": float(pw[-1, 1]),
     }

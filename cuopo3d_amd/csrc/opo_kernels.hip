@@ -139,6 +139,14 @@ __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
     nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds);
 }
 
+template <typename T, int NT>
+__global__ void __launch_bounds__(64) k_tline3r(TLine3Args<T> a, int nx, int ny,
+                                                const cplx<T> *__restrict__ tw, const int *status) {
+    if (status && *status) return;
+    __shared__ cplx<T> buf[RegTile<T, NT>::LDS];
+    nl_time_tile_reg<T, NT, false>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, buf);
+}
+
 template <typename T, int NT, int TX, int BS>
 __global__ void __launch_bounds__(BS) k_tline3p(TLine3Args<T> a, int nx, int ny,
                                                 const cplx<T> *__restrict__ tw, const int *status) {
@@ -288,7 +296,12 @@ static hipError_t tline3_launch_v(hipStream_t st, const Geometry &g, const TLine
     } else {
         if (g.nx % Vr::tx != 0) return hipErrorInvalidValue;
         const int units = (g.nx / Vr::tx) * g.ny;
-        if constexpr (Vr::persistent) {
+        if constexpr (Vr::reg) {
+            if (a.rk.has_phase) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_tline3r<T, NT>), dim3((unsigned)units), dim3(64), 0, st, a, g.nx, g.ny, tw,
+                               status);
+            return hipGetLastError();
+        } else if constexpr (Vr::persistent) {
             if (a.rk.has_phase) return hipErrorInvalidValue;
             hipLaunchKernelGGL((k_tline3p<T, NT, Vr::tx, Vr::bs>), dim3(persistent_grid(units, Vr::per_cu)),
                                dim3(Vr::bs), 0, st, a, g.nx, g.ny, tw, status);
