@@ -62,11 +62,6 @@ __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ s
     nl_time_tile<T, NT, TX, BS, false>(a, d.nx, d.ny, w.item, d.twt, lds);
 }
 
-template <typename T, int NT>
-__global__ void __launch_bounds__(64) kb_timer(const SolveDesc<T> *__restrict__ sd,
-                                               const int *__restrict__ status,
-                                               const WorkItem *__restrict__ work, int mode);
-
 // Work units of the fused pass over a work list: tile w.item of solve w.solve.
 template <typename T, int NT, int TX> struct TimeBatch {
     const SolveDesc<T> *sd;
@@ -102,19 +97,6 @@ template <typename T, int NT, int TX> struct TimeBatch {
         stage_time_tables<T, NT>(sd[work[u].solve].twt, args(u), stab, tid, nthr);
     }
 };
-
-template <typename T, int NT>
-__global__ void __launch_bounds__(64) kb_timer(const SolveDesc<T> *__restrict__ sd,
-                                               const int *__restrict__ status,
-                                               const WorkItem *__restrict__ work, int mode) {
-    __shared__ cplx<T> buf[RegTile<T, NT>::LDS];
-    const int u = xcd_local_tile(blockIdx.x, gridDim.x);
-    const WorkItem w = work[u];
-    if (status[w.solve]) return;
-    const TimeBatch<T, NT, RegTile<T, NT>::P> src{sd, status, work, (int)gridDim.x, mode};
-    const SolveDesc<T> &d = sd[w.solve];
-    nl_time_tile_reg<T, NT, false>(src.args(u), d.nx, d.ny, w.item, d.twt, buf);
-}
 
 template <typename T, int NT, int TX, int BS>
 __global__ void __launch_bounds__(BS) kb_timep(const SolveDesc<T> *__restrict__ sd,
@@ -265,10 +247,7 @@ static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
         using Vr = TimeVariant<T, NT, V>;
         if constexpr (Vr::valid) {
             if (V == v) {
-                if constexpr (Vr::reg)
-                    hipLaunchKernelGGL((kb_timer<T, NT>), dim3((unsigned)n_work), dim3(64), 0, st, sd, status, work,
-                                       mode);
-                else if constexpr (Vr::persistent)
+                if constexpr (Vr::persistent)
                     hipLaunchKernelGGL((kb_timep<T, NT, Vr::tx, Vr::bs>), dim3(persistent_grid(n_work, Vr::per_cu)),
                                        dim3(Vr::bs), 0, st, sd, status, work, n_work, mode);
                 else

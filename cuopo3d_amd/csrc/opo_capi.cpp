@@ -616,23 +616,42 @@ template <typename T> struct SolveT final : SolveImpl {
         if (rc) return rc;
         if (reps < 1) reps = 1;
         const bool fp64 = sizeof(T) == 8;
+        const int orig[2] = {tune_get(fp64, 0, g.nt), tune_get(fp64, 1, g.nx)};
         const size_t fb = N * sizeof(C);
         C *save[3] = {nullptr, nullptr, nullptr};
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        auto cleanup = [&]() {
-            for (auto *b : save) (void)hipFree(b);
-            if (e0) (void)hipEventDestroy(e0);
-            if (e1) (void)hipEventDestroy(e1);
-        };
-        for (int f = 0; f < 3; ++f) {
-            if (hipMalloc(&save[f], fb) != hipSuccess) {
-                cleanup();
-                return fail(OPO_ERR_NOMEM, "autotune: no room to save the fields");
-            }
-            CK(hipMemcpyAsync(save[f], A[f], fb, hipMemcpyDeviceToDevice, stream));
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        bool saved = false;
+        rc = autotune_run(reps, report, save, ev, saved);
+        // Whatever happened, put the fields back and release the scratch;
+        // on failure the previous choices stand.
+        if (saved)
+            for (int f = 0; f < 3; ++f)
+                (void)hipMemcpyAsync(A[f], save[f], fb, hipMemcpyDeviceToDevice, stream);
+        const hipError_t es = hipStreamSynchronize(stream);
+        for (auto *b : save) (void)hipFree(b);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (rc == OPO_OK && es != hipSuccess) rc = fail(OPO_ERR_HIP, "autotune: %s", hipGetErrorString(es));
+        if (rc != OPO_OK) {
+            tune_set(fp64, 0, g.nt, orig[0]);
+            tune_set(fp64, 1, g.nx, orig[1]);
         }
-        CK(hipEventCreate(&e0));
-        CK(hipEventCreate(&e1));
+        for (auto &gx : graph)   // re-capture with the chosen variants
+            if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+        return rc;
+    }
+
+    // The measurement of autotune(); scratch is owned by the caller.
+    int autotune_run(int reps, std::string &report, C *save[3], hipEvent_t ev[2], bool &saved) {
+        const bool fp64 = sizeof(T) == 8;
+        const size_t fb = N * sizeof(C);
+        for (int f = 0; f < 3; ++f)
+            if (hipMalloc(&save[f], fb) != hipSuccess)
+                return fail(OPO_ERR_NOMEM, "autotune: no room to save the fields");
+        for (int f = 0; f < 3; ++f) CK(hipMemcpyAsync(save[f], A[f], fb, hipMemcpyDeviceToDevice, stream));
+        saved = true;
+        CK(hipEventCreate(&ev[0]));
+        CK(hipEventCreate(&ev[1]));
         // Candidates are timed in context: a stretch of the slice schedule
         // ([t pass][xy pass] per slice), so each pass runs after the other
         // as it does in a round trip.  Coordinate descent: the t pass with
@@ -660,14 +679,15 @@ template <typename T> struct SolveT final : SolveImpl {
                 if (!tune_variant_valid(fp64, axis, n, v)) continue;
                 if (axis == 0 && (g.nx == 1 || g.nx % tune_tile_px(fp64, n, v) != 0)) continue;
                 tune_set(fp64, axis, n, v);
-                if ((rc = stretch())) { cleanup(); return rc; }
-                CK(hipEventRecord(e0, stream));
+                int rc = stretch();
+                if (rc) return rc;
+                CK(hipEventRecord(ev[0], stream));
                 for (int r = 0; r < reps && !rc; ++r) rc = stretch();
-                if (rc) { cleanup(); return rc; }
-                CK(hipEventRecord(e1, stream));
-                CK(hipEventSynchronize(e1));
+                if (rc) return rc;
+                CK(hipEventRecord(ev[1], stream));
+                CK(hipEventSynchronize(ev[1]));
                 float ms = 0.f;
-                CK(hipEventElapsedTime(&ms, e0, e1));
+                CK(hipEventElapsedTime(&ms, ev[0], ev[1]));
                 const double us = 1e3 * ms / ((double)reps * L);
                 const int pc = tune_per_cu(fp64, axis, n, v);
                 char pers[32] = "";
@@ -683,12 +703,6 @@ template <typename T> struct SolveT final : SolveImpl {
             tune_set(fp64, axis, n, best);
             report += " -> v" + std::to_string(best) + "\n";
         }
-        for (int f = 0; f < 3; ++f)
-            CK(hipMemcpyAsync(A[f], save[f], fb, hipMemcpyDeviceToDevice, stream));
-        CK(hipStreamSynchronize(stream));
-        cleanup();
-        for (auto &gx : graph)   // re-capture with the chosen variants
-            if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
         return OPO_OK;
     }
 

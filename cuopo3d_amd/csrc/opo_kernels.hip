@@ -13,6 +13,8 @@
 //   k_xyplane  Q_x then Q_y on a whole t-plane staged in LDS (rows padded
 //              by one element so column and row walks are bank-spread);
 //   k_power_*  compensated power sums with a fixed reduction order.
+#include <atomic>
+
 #include "opo_device.hpp"
 #include "opo_kernels.hpp"
 #include "opo_rk4.hpp"
@@ -139,14 +141,6 @@ __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
     nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds);
 }
 
-template <typename T, int NT>
-__global__ void __launch_bounds__(64) k_tline3r(TLine3Args<T> a, int nx, int ny,
-                                                const cplx<T> *__restrict__ tw, const int *status) {
-    if (status && *status) return;
-    __shared__ cplx<T> buf[RegTile<T, NT>::LDS];
-    nl_time_tile_reg<T, NT, false>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, buf);
-}
-
 template <typename T, int NT, int TX, int BS>
 __global__ void __launch_bounds__(BS) k_tline3p(TLine3Args<T> a, int nx, int ny,
                                                 const cplx<T> *__restrict__ tw, const int *status) {
@@ -158,16 +152,16 @@ __global__ void __launch_bounds__(BS) k_tline3p(TLine3Args<T> a, int nx, int ny,
 
 // Compute units of the current device (persistent grids are sized by it).
 static int cu_count() {
-    static int cached[64] = {0};
+    static std::atomic<int> cached[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (!cached[dev]) {
-        int n = 0;
+    int n = cached[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
         if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
             n = 256;
-        cached[dev] = n;
+        cached[dev].store(n, std::memory_order_relaxed);
     }
-    return cached[dev];
+    return n;
 }
 
 unsigned persistent_grid(int units, int per_cu) {
@@ -176,9 +170,24 @@ unsigned persistent_grid(int units, int per_cu) {
 }
 
 // ----------------------------------------------------- variant tables
-static int g_time_var[2][11];   // [fp64][log2 NT]
-static int g_xy_var[2][8];      // [fp64][NX index]
-static bool g_tune_init = false;
+// Chosen variant per (precision, size): process-wide, read at every launch
+// and written by the tuner, so the entries are atomics (a launch sees either
+// the old or the new choice, both of which give the same results).
+struct TuneTables {
+    std::atomic<int> time[2][11];   // [fp64][log2 NT]
+    std::atomic<int> xy[2][8];      // [fp64][NX index]
+    TuneTables() {
+        const int nxs[6] = {1, 16, 32, 64, 96, 128};
+        for (int f = 0; f < 2; ++f) {
+            for (int k = 0; k < 11; ++k) time[f][k].store(time_default_variant(f != 0, 1 << k));
+            for (int k = 0; k < 8; ++k) xy[f][k].store(k < 6 ? xy_default_variant(nxs[k]) : 0);
+        }
+    }
+};
+static TuneTables &tune_tables() {
+    static TuneTables tt;   // initialised once, thread-safe
+    return tt;
+}
 
 static int log2i(int n) {
     int l = 0;
@@ -190,15 +199,6 @@ static int nx_index(int nx) {
         case 1: return 0; case 16: return 1; case 32: return 2; case 64: return 3;
         case 96: return 4; case 128: return 5; default: return -1;
     }
-}
-static void tune_init() {
-    if (g_tune_init) return;
-    for (int f = 0; f < 2; ++f) {
-        for (int k = 0; k < 11; ++k) g_time_var[f][k] = time_default_variant(f != 0, 1 << k);
-        const int nxs[6] = {1, 16, 32, 64, 96, 128};
-        for (int k = 0; k < 6; ++k) g_xy_var[f][k] = xy_default_variant(nxs[k]);
-    }
-    g_tune_init = true;
 }
 
 template <typename T, int NT>
@@ -250,20 +250,20 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
 }
 
 int tune_get(bool fp64, int axis, int n) {
-    tune_init();
+    TuneTables &tt = tune_tables();
     if (axis == 0) {
         const int l = log2i(n);
-        return (l >= 0 && l < 11) ? g_time_var[fp64][l] : 0;
+        return (l >= 0 && l < 11) ? tt.time[fp64][l].load(std::memory_order_relaxed) : 0;
     }
     const int k = nx_index(n);
-    return k >= 0 ? g_xy_var[fp64][k] : 0;
+    return k >= 0 ? tt.xy[fp64][k].load(std::memory_order_relaxed) : 0;
 }
 
 bool tune_set(bool fp64, int axis, int n, int v) {
-    tune_init();
     if (!tune_variant_valid(fp64, axis, n, v)) return false;
-    if (axis == 0) g_time_var[fp64][log2i(n)] = v;
-    else g_xy_var[fp64][nx_index(n)] = v;
+    TuneTables &tt = tune_tables();
+    if (axis == 0) tt.time[fp64][log2i(n)].store(v, std::memory_order_relaxed);
+    else tt.xy[fp64][nx_index(n)].store(v, std::memory_order_relaxed);
     return true;
 }
 
@@ -296,12 +296,7 @@ static hipError_t tline3_launch_v(hipStream_t st, const Geometry &g, const TLine
     } else {
         if (g.nx % Vr::tx != 0) return hipErrorInvalidValue;
         const int units = (g.nx / Vr::tx) * g.ny;
-        if constexpr (Vr::reg) {
-            if (a.rk.has_phase) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((k_tline3r<T, NT>), dim3((unsigned)units), dim3(64), 0, st, a, g.nx, g.ny, tw,
-                               status);
-            return hipGetLastError();
-        } else if constexpr (Vr::persistent) {
+        if constexpr (Vr::persistent) {
             if (a.rk.has_phase) return hipErrorInvalidValue;
             hipLaunchKernelGGL((k_tline3p<T, NT, Vr::tx, Vr::bs>), dim3(persistent_grid(units, Vr::per_cu)),
                                dim3(Vr::bs), 0, st, a, g.nx, g.ny, tw, status);

@@ -38,16 +38,6 @@ template <typename T, int NT, int TX> struct TimeTile {
     __device__ static __forceinline__ int at(int i) { return i + (i / (N2 * TX)) * PAD; }
 };
 
-// Register-line layout (nl_time_tile_reg): N1 samples per lane, N2 lanes
-// per pixel, P = 64 / N2 pixels per wave.
-template <typename T, int NT> struct RegTile {
-    static constexpr int N1 = Split<NT>::N1, N2 = Split<NT>::N2;
-    static constexpr bool ok = N2 > 1 && 64 % N2 == 0 && N1 % N2 == 0 && N1 <= 16;
-    static constexpr int P = ok ? 64 / N2 : 1;     // pixels per wave (the tile width)
-    static constexpr int RP = N2 + 1;              // padded exchange row
-    static constexpr int LDS = P * N1 * RP;        // elements of the exchange buffer
-};
-
 // Launch variants of the fused pass for one NT (chosen by the autotuner).
 // V < 9: one tile per workgroup, V = 3 m + b with tile width m = 0: TXd,
 // 1: TXd/2, 2: 2 TXd pixels and b = 0: 256, 1: 512, 2: 384 threads (384
@@ -55,29 +45,27 @@ template <typename T, int NT> struct RegTile {
 // tiles).  V >= 9: persistent workgroups (k + 1 per CU), each walking a
 // contiguous run of tiles with the next tile prefetched into registers;
 // V - 9 = 2 (3 m + b) + k with b = 0: 256, 1: 384, 2: 512 threads.
-// V = 27: register lines (one wave per workgroup, nl_time_tile_reg), for
-// NT = 64, 128, 256.  Only variant 0 exists for FP64 and for NT < 64; a
-// variant whose three tiles exceed 112 KiB of LDS (persistent: tiles plus
-// staged tables beyond 144 KiB) does not exist.
+// Only variant 0 exists for FP64 and for NT < 64; a variant whose three
+// tiles exceed 112 KiB of LDS (persistent: tiles plus staged tables beyond
+// 144 KiB) does not exist.  (A form that keeps whole t-lines in one wave's
+// registers, exchanging through wave-private LDS, was measured 15-40 %
+// slower than the LDS tiles on every grid and is not offered.)
 constexpr int kTimeClassic = 9;
-constexpr int kTimeRegLines = kTimeClassic + 18;   // the register-line variant
-constexpr int kTimeVariants = kTimeRegLines + 1;
+constexpr int kTimeVariants = kTimeClassic + 18;
 template <typename T, int NT, int V> struct TimeVariant {
-    static constexpr bool reg = V == kTimeRegLines;
     static constexpr int txd = tline3_tx<T, NT>();
-    static constexpr bool persistent = V >= kTimeClassic && !reg;
+    static constexpr bool persistent = V >= kTimeClassic;
     static constexpr int idx = persistent ? V - kTimeClassic : V;
     static constexpr int m = persistent ? idx / 6 : idx / 3;
     static constexpr int b = persistent ? (idx / 2) % 3 : idx % 3;
     static constexpr int per_cu = persistent ? idx % 2 + 1 : 0;
-    static constexpr int tx = reg ? RegTile<T, NT>::P : (m == 0 ? txd : (m == 1 ? txd / 2 : txd * 2));
-    static constexpr int bs = reg ? 64 : (persistent ? (b == 0 ? 256 : (b == 1 ? 384 : 512)) : (b == 0 ? 256 : (b == 1 ? 512 : 384)));
+    static constexpr int tx = m == 0 ? txd : (m == 1 ? txd / 2 : txd * 2);
+    static constexpr int bs = persistent ? (b == 0 ? 256 : (b == 1 ? 384 : 512)) : (b == 0 ? 256 : (b == 1 ? 512 : 384));
     static constexpr int data_bytes = 3 * TimeTile<T, NT, (tx > 0 ? tx : 1)>::FT * (int)sizeof(cplx<T>);
     static constexpr int table_bytes = 8 * NT * (int)sizeof(cplx<T>);  // twiddles + 3 tables, hi/lo
     static constexpr bool valid =
-        V == 0 || (reg ? (sizeof(T) == 4 && RegTile<T, NT>::ok)
-                       : (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 114688 &&
-                          (!persistent || data_bytes + table_bytes <= 147456)));
+        V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 114688 &&
+                   (!persistent || data_bytes + table_bytes <= 147456));
 };
 // Untuned choice: the default tile with 512 threads where it exists.
 constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 64) ? 1 : 0; }
@@ -186,17 +174,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
             Tri<T> A = {a.psrc[gi], a.f[1][gi], a.f[2][gi]};
-            if (rk.nsteps > 0) rk4_half_step<T, PHASE>(rk, 0, A);
-            if (rk.nsteps > 1) rk4_half_step<T, PHASE>(rk, 1, A);
-            if (rk.out_scale) {
-                if (rk.eom) {
-                    const cplx<T> e = rk.eom[2 * t], el = rk.eom[2 * t + 1];
-                    A.s = cmul_hl(A.s, e, el);
-                    A.i = cmul_hl(A.i, e, el);
-                }
-                A.s = cmul_hl(A.s, cplx<T>{rk.ms_re, rk.ms_im}, cplx<T>{rk.ms_lo_re, rk.ms_lo_im});
-                A.i = cmul_hl(A.i, cplx<T>{rk.mi_re, rk.mi_im}, cplx<T>{rk.mi_lo_re, rk.mi_lo_im});
-            }
+            nl_point<T, PHASE>(rk, t, A);
             const int li = L::at(i);
             lds[li] = A.p;
             lds[FT + li] = A.s;
@@ -224,93 +202,6 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     }
 }
 
-
-// Register-line form of the fused pass for NT = N1 x N2 with N2 <= N1 <= 16
-// (NT = 64, 128, 256): one wave per workgroup holds the whole t-lines of
-// P = 64 / N2 consecutive pixels of a row, lane (p, n2) keeping samples
-// t = n1 N2 + n2 (n1 < N1) of the three fields.  The pointwise step and the
-// N1-point DFTs run in registers; only the two exchanges of each line
-// operator pass through a wave-private LDS buffer, so no barrier waits on
-// another wave.  The arithmetic is the line operator's (line_op_set) step
-// for step, so the result equals the LDS-tiled pass bit for bit.
-
-template <typename T, int NT, bool PHASE>
-__device__ __forceinline__ void nl_time_tile_reg(const TLine3Args<T> &a, int nx, int ny, int tile,
-                                                 const cplx<T> *__restrict__ tw, cplx<T> *buf) {
-    using R = RegTile<T, NT>;
-    static_assert(R::ok, "register-line split");
-    constexpr int N1 = R::N1, N2 = R::N2, P = R::P, RP = R::RP;
-    const int lane = threadIdx.x;
-    const int px = lane / N2, n2 = lane % N2;
-    const int tiles_x = nx / P;
-    const size_t plane = (size_t)nx * ny;
-    const size_t pix = (size_t)(tile / tiles_x) * nx + (tile % tiles_x) * P + px;
-    cplx<T> v[3][N1];
-#pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) {
-        const size_t gi = (size_t)(n1 * N2 + n2) * plane + pix;
-        v[0][n1] = a.psrc[gi];
-        v[1][n1] = a.f[1][gi];
-        v[2][n1] = a.f[2][gi];
-    }
-#pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) {
-        Tri<T> A = {v[0][n1], v[1][n1], v[2][n1]};
-        nl_point<T, PHASE>(a.rk, n1 * N2 + n2, A);
-        v[0][n1] = A.p;
-        v[1][n1] = A.s;
-        v[2][n1] = A.i;
-    }
-    cplx<T> *mine = buf + px * N1 * RP;
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        if (f >= a.first_field) {
-            const cplx<T> *tab = a.tab0 + (size_t)(f - a.first_field) * a.tab_stride;
-            // phase 1: N1-point DFTs (exponent +1) and the inter-level twiddles
-            dft<N1, +1>(v[f]);
-#pragma unroll
-            for (int k1 = 1; k1 < N1; ++k1) {
-                const int j = n2 * k1;
-                v[f][k1] = cmulc_hl(v[f][k1], tw[2 * j], tw[2 * j + 1]);
-            }
-#pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) mine[k1 * RP + n2] = v[f][k1];
-            __syncthreads();
-            // phase 2: row k1 = n2 + r N2 of this pixel
-#pragma unroll
-            for (int r = 0; r < N1 / N2; ++r) {
-                const int k1 = n2 + r * N2;
-                cplx<T> w[N2];
-                cplx<T> *row = mine + k1 * RP;
-#pragma unroll
-                for (int q = 0; q < N2; ++q) w[q] = row[q];
-                dft<N2, +1>(w);
-#pragma unroll
-                for (int k2 = 0; k2 < N2; ++k2) {
-                    const int k = k1 + N1 * k2;
-                    w[k2] = cmul_hl(w[k2], tab[2 * k], tab[2 * k + 1]);
-                }
-                dft<N2, -1>(w);
-#pragma unroll
-                for (int q = 1; q < N2; ++q) {
-                    const int j = q * k1;
-                    w[q] = cmul_hl(w[q], tw[2 * j], tw[2 * j + 1]);
-                }
-#pragma unroll
-                for (int q = 0; q < N2; ++q) row[q] = w[q];
-            }
-            __syncthreads();
-            // phase 1': inverse N1-point DFTs back to time samples
-#pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) v[f][k1] = mine[k1 * RP + n2];
-            dft<N1, -1>(v[f]);
-            __syncthreads();   // the buffer is reused by the next field
-        }
-        cplx<T> *out = a.f[f];
-#pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) out[(size_t)(n1 * N2 + n2) * plane + pix] = v[f][n1];
-    }
-}
 
 // Contiguous run [u0, u1) of the work units of this workgroup.
 __device__ __forceinline__ void unit_run(int units, int &u0, int &u1) {
