@@ -50,6 +50,12 @@ def test_create_rejects_bad_input():
     bad = p.copy()
     bad.dz = 0.0
     assert lib.opo_create(C.byref(g), C.byref(bad), 0, C.byref(h)) == OPO_ERR_ARG
+    bad = p.copy()   # the modulator needs the time window of its time grid
+    bad.eom_on, bad.eom_beta, bad.eom_fpm = 1, 0.5, 0.1
+    bad.t_window = 0.0
+    bad.omega = capi.C.cast(capi.C.pointer((capi.C.c_double * 64)()), capi.C.POINTER(capi.C.c_double))
+    assert lib.opo_create(C.byref(g), C.byref(bad), 0, C.byref(h)) == OPO_ERR_ARG
+    assert b"t_window" in lib.opo_last_error()
     if lib.opo_device_count() == 0:
         assert lib.opo_create(C.byref(g), C.byref(p), 0, C.byref(h)) == OPO_ERR_NOGPU
     assert not h.value

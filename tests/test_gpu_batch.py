@@ -194,3 +194,39 @@ def test_cpp_driver_runs(tmp_path, require_gpu):
     assert rows.shape == (3, 4) and np.all(np.isfinite(rows))
     assert (tmp_path / "OutputSignalPower_PumpPower500_W_slice32_r.dat").exists()
     assert (tmp_path / "As_temporal_x_16_y_16_i.dat").exists()
+
+
+def test_fp64_batch_equals_single_solves(require_gpu):
+    """An FP64 batch (all-FP64 device path) gives the FP64 single-solve
+    fields bitwise and the oracle's to 1e-11."""
+    from cuopo3d_amd.capi import OPO_FP64
+    solves = []
+    for n in ((32, 32, 64), (64, 64, 32)):
+        g, p, ph, f = configs.cavity_64(n=n, n_round_trips=2, precision=OPO_FP64)
+        solves.append((g, p, ph, f))
+    b = co.Batch(4 * GB, precision=OPO_FP64)
+    tickets = [b.submit(g, p, *f, 2) for (g, p, ph, f) in solves]
+    res = {}
+    for _ in range(20):
+        b.step()
+        for t in tickets:
+            if t not in res:
+                r = b.poll(t)
+                if r is not None:
+                    res[t] = r
+        if len(res) == len(tickets):
+            break
+    b.close()
+    for (g, p, ph, f), t in zip(solves, tickets):
+        pb, fb = res[t]
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        ps = s.runOPO(2)
+        for a, c in zip(s.get_fields(), fb):
+            assert np.array_equal(a, c)
+        o = Oracle(g, p)
+        o.set_fields(*f)
+        o.run_round_trips(2)
+        errs = [rel_l2(a, c) for a, c in zip(fb, o.get_fields())]
+        print((g.nx, g.nt), "fp64 batch vs oracle", errs, flush=True)
+        assert max(errs) < 1e-11
