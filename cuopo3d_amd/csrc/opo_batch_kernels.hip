@@ -272,6 +272,7 @@ hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc
         case 256: return bt_dispatch<T, 256>(st, n_work, sd, status, work, mode);
         case 512: return bt_dispatch<T, 512>(st, n_work, sd, status, work, mode);
         case 1024: return bt_dispatch<T, 1024>(st, n_work, sd, status, work, mode);
+        case 2048: return bt_dispatch<T, 2048>(st, n_work, sd, status, work, mode);
         default: return hipErrorInvalidValue;
     }
 }

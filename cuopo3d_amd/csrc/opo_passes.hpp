@@ -149,9 +149,10 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     const size_t plane = (size_t)nx * ny;
     const size_t base = (size_t)y * nx + x0;
     const Rk4Args<T> &rk = a.rk;
-    // pointwise step in registers, two pixels per thread per iteration (a
-    // version that issued the loads of four iterations ahead of the
-    // arithmetic was measured slower: more registers, no latency gained)
+    // pointwise step in registers, two pixels per thread per iteration
+    // (versions that issued the loads of one or of four iterations ahead of
+    // the arithmetic were measured no faster: more registers, no latency
+    // gained)
     if constexpr (TX % 2 == 0) {
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const int t = i / TX, xl = i % TX;

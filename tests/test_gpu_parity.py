@@ -18,7 +18,8 @@ from helpers import rel_l2, random_fields, cavity_params
 pytestmark = pytest.mark.gpu
 
 OP_GRIDS = [(16, 16, 16), (32, 32, 64), (64, 64, 32), (96, 96, 16), (128, 128, 16),
-            (16, 16, 128), (16, 16, 256), (16, 16, 512), (16, 16, 1024), (1, 1, 64)]
+            (16, 16, 128), (16, 16, 256), (16, 16, 512), (16, 16, 1024), (16, 16, 2048),
+            (1, 1, 64)]
 
 
 def _pair(grid, p, fields):
@@ -112,6 +113,31 @@ def test_round_trips_small_cavity(require_gpu):
     _check(s, o, 1e-5, "4 trips 32x32x64")
     rel = np.abs(pg - pr) / np.abs(pr)
     assert rel.max() < 1e-6, rel
+
+
+def test_round_trips_nt2048(require_gpu):
+    """NT = 2048, the time grid the reference compiles by default, on a
+    32x32 transverse grid: 2 trips, fields and per-trip power."""
+    grid, p, ph, fields = configs.cavity_64(n=(32, 32, 2048), n_round_trips=2)
+    s, o = _pair(grid, p, fields)
+    pg = s.runOPO(2)
+    pr = o.run_round_trips(2)
+    rel = np.abs(pg - pr) / np.abs(pr)
+    print("NT=2048 per-trip power rel err (p,s,i):", rel.max(axis=0), flush=True)
+    _check(s, o, 1e-5, "2 trips 32x32x2048")
+    assert rel.max() < 1e-6, rel
+
+
+@pytest.mark.parametrize("n", [(16, 16, 2048), (32, 32, 1024)], ids=lambda n: "x".join(map(str, n)))
+def test_fp64_long_lines(n, require_gpu):
+    """FP64 fused passes on the longest time lines (their line operators
+    keep part of their registers in scratch): one trip at 1e-11."""
+    grid, p, ph, fields = configs.cavity_64(precision=OPO_FP64, n=n, n_round_trips=1)
+    s, o = _pair(grid, p, fields)
+    pg = s.runOPO(1)
+    pr = o.run_round_trips(1)
+    _check(s, o, 1e-11, f"fp64 1 trip {n}")
+    assert (np.abs(pg - pr) / np.abs(pr)).max() < 1e-11
 
 
 def test_fp64_single_pass(require_gpu):

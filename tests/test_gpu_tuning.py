@@ -17,14 +17,14 @@ from helpers import rel_l2, batch_mix, batch_run
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(64, 64, 256), (32, 32, 1024), (128, 128, 64), (16, 16, 512)]
+GRIDS = [(64, 64, 256), (32, 32, 1024), (128, 128, 64), (16, 16, 512), (16, 16, 2048)]
 
 
 @pytest.fixture
 def restore_tuning():
     lib = co.load()
     saved = {}
-    for nt in (16, 32, 64, 128, 256, 512, 1024):
+    for nt in (16, 32, 64, 128, 256, 512, 1024, 2048):
         saved[(OPO_TUNE_TIME_PASS, nt)] = lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, nt)
     for nx in (16, 32, 64, 96, 128):
         saved[(OPO_TUNE_XY_PASS, nx)] = lib.opo_tune_get(OPO_FP32, OPO_TUNE_XY_PASS, nx)
