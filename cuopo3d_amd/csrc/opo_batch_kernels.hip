@@ -135,6 +135,13 @@ template <typename T, int NX> struct XyBatch {
         const cplx<T> *ty[3] = {by, by + 2 * NX, by + 4 * NX};
         stage_xy_tables<T, NX>(d.twx, tx, ty, free_space ? 2 : 3, stab, tid, nthr);
     }
+    __device__ void tables(int u, const cplx<T> *&tw, const cplx<T> *&tx, const cplx<T> *&ty) const {
+        const SolveDesc<T> &d = sd[work[u].solve];
+        const int k = work[u].item / d.nt;
+        tw = d.twx;
+        tx = (free_space ? d.qfsx : d.qx) + 2 * NX * k;
+        ty = (free_space ? d.qfsy : d.qy) + 2 * NX * k;
+    }
 };
 
 template <typename T, int NX, int BS>
@@ -143,7 +150,7 @@ __global__ void __launch_bounds__(BS) kb_xyp(const SolveDesc<T> *__restrict__ sd
                                              const int *__restrict__ status,
                                              const WorkItem *__restrict__ work, int n_work, int free_space) {
     __shared__ cplx<T> lds[NX * (NX + 1)];
-    __shared__ cplx<T> stab[14 * NX];
+    __shared__ cplx<T> stab[xy_uniform<NX, BS>() ? 1 : 14 * NX];
     xy_run<T, NX, BS>(XyBatch<T, NX>{sd, ts, status, work, n_work, free_space}, lds, stab);
 }
 

@@ -287,12 +287,34 @@ template <typename T> struct LineSet {
 // (element n sits at n*ES + (n / N2)*PAD), so that the N1 rows walked in
 // phase 2 start in different LDS banks when N2*ES is a multiple of the
 // bank count.
-template <typename T, int N, int S, int LS, int ES, int PAD = 0>
+// UNI: the caller guarantees nlines and nthr are multiples of the wave
+// size, so the 64 tasks of a wave share their (sub-index, field) and every
+// twiddle and table entry a wave reads is one address: the index is made
+// wave-uniform and the constants come through the scalar cache instead of
+// 64 vector lanes each.
+template <typename T> using const_space_ptr = const __attribute__((address_space(4))) T *;
+
+// Element i of a read-only table.  With UNI the table is read through the
+// constant address space: the kernel never writes it, and a wave-uniform
+// index then becomes a scalar load even where the kernel stores to global
+// memory between two reads (which otherwise forces vector loads).
+template <typename T, bool UNI>
+__device__ __forceinline__ cplx<T> table_at(const cplx<T> *__restrict__ p, int i) {
+    if constexpr (UNI) {
+        const const_space_ptr<T> q = (const_space_ptr<T>)p;
+        return cplx<T>{q[2 * i], q[2 * i + 1]};
+    } else {
+        return p[i];
+    }
+}
+
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false>
 __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
                                             const cplx<T> *__restrict__ tw, int tid, int nthr) {
     constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
     constexpr int RS = N2 * ES + PAD;   // stride between the N1 rows of N2 elements
     static_assert(N2 > 1 || PAD == 0, "padding applies to two-level lines");
+    auto uni = [](int r) { return UNI ? __builtin_amdgcn_readfirstlane(r) : r; };
     if constexpr (N2 == 1) {
         for (int task = tid; task < nlines * ls.nf; task += nthr) {
             const int l = task % nlines, f = task / nlines;
@@ -303,7 +325,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
             for (int n = 0; n < N; ++n) v[n] = base[n * ES];
             dft<N, S>(v);
 #pragma unroll
-            for (int k = 0; k < N; ++k) v[k] = cmul_hl(v[k], tab[2 * k], tab[2 * k + 1]);
+            for (int k = 0; k < N; ++k) v[k] = cmul_hl(v[k], table_at<T, UNI>(tab, 2 * k), table_at<T, UNI>(tab, 2 * k + 1));
             dft<N, -S>(v);
 #pragma unroll
             for (int n = 0; n < N; ++n) base[n * ES] = v[n];
@@ -312,7 +334,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
     } else {
         // phase 1: task (l, n2, f)
         for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
-            const int l = task % nlines, r = task / nlines, n2 = r % N2, f = r / N2;
+            const int l = task % nlines, r = uni(task / nlines), n2 = r % N2, f = r / N2;
             cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
             cplx<T> v[N1];
 #pragma unroll
@@ -321,8 +343,8 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
 #pragma unroll
             for (int k1 = 1; k1 < N1; ++k1) {
                 const int j = n2 * k1;
-                v[k1] = S < 0 ? cmul_hl(v[k1], tw[2 * j], tw[2 * j + 1])
-                              : cmulc_hl(v[k1], tw[2 * j], tw[2 * j + 1]);
+                v[k1] = S < 0 ? cmul_hl(v[k1], table_at<T, UNI>(tw, 2 * j), table_at<T, UNI>(tw, 2 * j + 1))
+                              : cmulc_hl(v[k1], table_at<T, UNI>(tw, 2 * j), table_at<T, UNI>(tw, 2 * j + 1));
             }
 #pragma unroll
             for (int k1 = 0; k1 < N1; ++k1) base[k1 * RS] = v[k1];
@@ -330,7 +352,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
         __syncthreads();
         // phase 2: task (l, k1, f)
         for (int task = tid; task < nlines * N1 * ls.nf; task += nthr) {
-            const int l = task % nlines, r = task / nlines, k1 = r % N1, f = r / N1;
+            const int l = task % nlines, r = uni(task / nlines), k1 = r % N1, f = r / N1;
             const cplx<T> *tab = ls.tab(f);
             cplx<T> *base = ls.base(f) + l * LS + k1 * RS;
             cplx<T> v[N2];
@@ -340,14 +362,14 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) {
                 const int k = k1 + N1 * k2;
-                v[k2] = cmul_hl(v[k2], tab[2 * k], tab[2 * k + 1]);
+                v[k2] = cmul_hl(v[k2], table_at<T, UNI>(tab, 2 * k), table_at<T, UNI>(tab, 2 * k + 1));
             }
             dft<N2, -S>(v);
 #pragma unroll
             for (int n2 = 1; n2 < N2; ++n2) {
                 const int j = n2 * k1;
-                v[n2] = S < 0 ? cmulc_hl(v[n2], tw[2 * j], tw[2 * j + 1])
-                              : cmul_hl(v[n2], tw[2 * j], tw[2 * j + 1]);
+                v[n2] = S < 0 ? cmulc_hl(v[n2], table_at<T, UNI>(tw, 2 * j), table_at<T, UNI>(tw, 2 * j + 1))
+                              : cmul_hl(v[n2], table_at<T, UNI>(tw, 2 * j), table_at<T, UNI>(tw, 2 * j + 1));
             }
 #pragma unroll
             for (int n2 = 0; n2 < N2; ++n2) base[n2 * ES] = v[n2];
@@ -355,7 +377,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
         __syncthreads();
         // phase 1': task (l, n2, f)
         for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
-            const int l = task % nlines, r = task / nlines, n2 = r % N2, f = r / N2;
+            const int l = task % nlines, r = uni(task / nlines), n2 = r % N2, f = r / N2;
             cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
             cplx<T> v[N1];
 #pragma unroll
@@ -369,11 +391,11 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
 }
 
 // One field.
-template <typename T, int N, int S, int LS, int ES>
+template <typename T, int N, int S, int LS, int ES, bool UNI = false>
 __device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> *__restrict__ tw,
                                         const cplx<T> *__restrict__ tab, int tid, int nthr) {
     LineSet<T> ls{lds, 0, tab, 0, 1};
-    line_op_set<T, N, S, LS, ES>(ls, nlines, tw, tid, nthr);
+    line_op_set<T, N, S, LS, ES, 0, UNI>(ls, nlines, tw, tid, nthr);
 }
 
 }  // namespace opo

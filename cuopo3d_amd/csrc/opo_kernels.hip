@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(BS) k_xyplanep(FieldOps<T> ops, int nt, int nf
                                                  const cplx<T> *__restrict__ tw, const int *status) {
     if (status && *status) return;
     __shared__ cplx<T> lds[NX * (NX + 1)];
-    __shared__ cplx<T> stab[14 * NX];
+    __shared__ cplx<T> stab[xy_uniform<NX, BS>() ? 1 : 14 * NX];
     xy_run<T, NX, BS>(XySolo<T, NX>{ops, nt, nf, tw}, lds, stab);
 }
 

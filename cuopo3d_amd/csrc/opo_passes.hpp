@@ -47,9 +47,10 @@ template <typename T, int NT, int TX> struct TimeTile {
 // V - 9 = 2 (3 m + b) + k with b = 0: 256, 1: 384, 2: 512 threads.
 // Only variant 0 exists for FP64 and for NT < 64; a variant whose three
 // tiles exceed 112 KiB of LDS (persistent: tiles plus staged tables beyond
-// 144 KiB) does not exist.  (A form that keeps whole t-lines in one wave's
-// registers, exchanging through wave-private LDS, was measured 15-40 %
-// slower than the LDS tiles on every grid and is not offered.)
+// 144 KiB) does not exist.  (Not offered, measured slower or no faster on
+// every grid: a form that keeps whole t-lines in one wave's registers,
+// exchanging through wave-private LDS, 15-40 % slower; 768 and 1024
+// threads per tile, i.e. three and four waves per SIMD.)
 constexpr int kTimeClassic = 9;
 constexpr int kTimeVariants = kTimeClassic + 18;
 template <typename T, int NT, int V> struct TimeVariant {
@@ -74,9 +75,9 @@ constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 6
 // 1024 threads; V >= 3 persistent workgroups (k + 1 per CU) walking a run
 // of planes with the next plane prefetched into registers,
 // V - 3 = 2 b + k with the same block sizes.  Persistent variants need at
-// most 16 prefetched elements per thread.  (All three fields of one time
-// sample per workgroup was measured slower than one plane per workgroup
-// on every grid and is not offered.)
+// most 16 prefetched elements per thread.  (Not offered, measured no
+// faster: all three fields of one time sample per workgroup; 512 threads
+// held to 80 registers so that three 64 x 64 planes share a CU.)
 constexpr int kXyVariants = 9;
 constexpr int xy_variant_bs(int v) {
     const int b = v < 3 ? v : (v - 3) / 2;
@@ -321,6 +322,10 @@ template <typename T, int NT, int TX> struct TimeSolo {
     }
 };
 
+// Whether the xy line operators of an NX plane on BS threads read their
+// constants wave-uniformly (through the scalar cache).
+template <int NX, int BS> constexpr bool xy_uniform() { return NX % 64 == 0 && BS % 64 == 0; }
+
 // Q_x then Q_y on the NX x NX plane A (x fastest).  lds holds NX*(NX+1)
 // elements: rows padded by one element so row and column walks spread over
 // the LDS banks.
@@ -333,20 +338,24 @@ __device__ __forceinline__ void xy_plane(cplx<T> *A, const cplx<T> *__restrict__
     for (int i = tid; i < NX * NX; i += BS) lds[(i / NX) * NXP + (i % NX)] = A[i];
     __syncthreads();
     // Q_x: lines are rows (pitch NXP), elements contiguous.
-    line_op<T, NX, -1, NXP, 1>(lds, NX, tw, tabx, tid, BS);
+    constexpr bool UNI = xy_uniform<NX, BS>();
+    line_op<T, NX, -1, NXP, 1, UNI>(lds, NX, tw, tabx, tid, BS);
     // Q_y: lines are columns, elements one row apart.
-    line_op<T, NX, -1, 1, NXP>(lds, NX, tw, taby, tid, BS);
+    line_op<T, NX, -1, 1, NXP, UNI>(lds, NX, tw, taby, tid, BS);
     for (int i = tid; i < NX * NX; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
 }
 
 // Persistent form of the xy pass: the workgroup walks a run of planes and
-// loads plane u+1 into registers while plane u is transformed.  stab holds
-// 14 NX staged elements: twiddles, then the x tables of three fields, then
-// their y tables.  Src supplies units(), skip(u), key(u), plane(u) (the
-// plane's first element), slot(u) (which staged table pair it uses) and
-// stage(u, stab, tid, nthr).
+// loads plane u+1 into registers while plane u is transformed.  Where the
+// constants are read wave-uniformly they come from global memory through
+// the scalar cache; otherwise stab holds 14 NX staged elements: twiddles,
+// then the x tables of three fields, then their y tables.  Src supplies
+// units(), skip(u), key(u), plane(u) (the plane's first element), slot(u)
+// (which staged table pair it uses), stage(u, stab, tid, nthr) and
+// tables(u, tw, tx, ty) (the global twiddles and tables of unit u).
 template <typename T, int NX, int BS, typename Src>
 __device__ __forceinline__ void xy_run(const Src &src, cplx<T> *lds, cplx<T> *stab) {
+    constexpr bool UNI = xy_uniform<NX, BS>();
     constexpr int NXP = NX + 1, NN = NX * NX;
     constexpr int PT = (NN + BS - 1) / BS;
     constexpr bool EVEN = NN % BS == 0;
@@ -370,7 +379,7 @@ __device__ __forceinline__ void xy_run(const Src &src, cplx<T> *lds, cplx<T> *st
             if (u + 1 < u1) load(u + 1);
             continue;
         }
-        if (src.key(u) != staged) {
+        if (!UNI && src.key(u) != staged) {
             src.stage(u, stab, tid, BS);
             staged = src.key(u);
         }
@@ -381,9 +390,16 @@ __device__ __forceinline__ void xy_run(const Src &src, cplx<T> *lds, cplx<T> *st
         }
         __syncthreads();
         if (u + 1 < u1) load(u + 1);
-        const int k = src.slot(u);
-        line_op<T, NX, -1, NXP, 1>(lds, NX, stab, stab + 2 * NX * (1 + k), tid, BS);
-        line_op<T, NX, -1, 1, NXP>(lds, NX, stab, stab + 2 * NX * (4 + k), tid, BS);
+        if constexpr (UNI) {
+            const cplx<T> *tw, *tx, *ty;
+            src.tables(u, tw, tx, ty);
+            line_op<T, NX, -1, NXP, 1, true>(lds, NX, tw, tx, tid, BS);
+            line_op<T, NX, -1, 1, NXP, true>(lds, NX, tw, ty, tid, BS);
+        } else {
+            const int k = src.slot(u);
+            line_op<T, NX, -1, NXP, 1>(lds, NX, stab, stab + 2 * NX * (1 + k), tid, BS);
+            line_op<T, NX, -1, 1, NXP>(lds, NX, stab, stab + 2 * NX * (4 + k), tid, BS);
+        }
         cplx<T> *A = src.plane(u);
         for (int i = tid; i < NN; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
         __syncthreads();
@@ -419,6 +435,12 @@ template <typename T, int NX> struct XySolo {
     }
     __device__ void stage(int, cplx<T> *stab, int tid, int nthr) const {
         stage_xy_tables<T, NX>(tw, ops.tab, ops.taby, nf, stab, tid, nthr);
+    }
+    __device__ void tables(int u, const cplx<T> *&tw_, const cplx<T> *&tx, const cplx<T> *&ty) const {
+        const int f = u / nt;
+        tw_ = tw;
+        tx = f == 0 ? ops.tab[0] : (f == 1 ? ops.tab[1] : ops.tab[2]);
+        ty = f == 0 ? ops.taby[0] : (f == 1 ? ops.taby[1] : ops.taby[2]);
     }
 };
 

@@ -37,7 +37,7 @@ def _valid(lib, axis, n):
     """Variants that exist for size n."""
     keep = lib.opo_tune_get(OPO_FP32, axis, n)
     out = []
-    for v in range(32):
+    for v in range(64):
         if lib.opo_tune_set(OPO_FP32, axis, n, v) == OPO_OK:
             out.append(v)
     lib.opo_tune_set(OPO_FP32, axis, n, keep)
