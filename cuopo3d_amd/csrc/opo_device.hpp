@@ -252,10 +252,11 @@ template <int N, int S, typename T> __device__ __forceinline__ void dft(cplx<T> 
 }
 
 // ----------------------------------------------------- line operator
-// Two-level split of each line length N into N1 * N2 (N1, N2 <= 32).
+// Two-level split of each line length N into N1 * N2 (N1, N2 <= 32); the
+// 2048-sample lines use the three-level split (Split3) instead.
 template <int N> struct Split {
     static constexpr int N1 = (N <= 32) ? N : (N == 64) ? 8 : (N == 96) ? 12 : (N == 128) ? 16
-                            : (N == 256) ? 16 : (N == 512) ? 32 : (N == 1024) ? 32 : (N == 2048) ? 64 : 0;
+                            : (N == 256) ? 16 : (N == 512) ? 32 : (N == 1024) ? 32 : 0;
     static constexpr int N2 = N / N1;
     static_assert(N1 * N2 == N, "unsupported line length");
 };
@@ -390,12 +391,147 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
     }
 }
 
+// Three-level split of long lines, N = N1 * N2 * N3 (M = N2 * N3): every
+// phase works on register DFTs of at most 16 points, so a tile of a few
+// pixels still gives each thread a task in every phase, at the price of two
+// more LDS exchanges and one more pair of twiddle multiplies.
+template <int N> struct Split3 {
+    static constexpr int N1 = (N == 512) ? 8 : (N == 1024) ? 16 : (N == 2048) ? 16 : 0;
+    static constexpr int N2 = (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 16 : 0;
+    static constexpr int N3 = (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 8 : 0;
+    static constexpr int M = N2 * N3;
+    static_assert(N1 * N2 * N3 == N, "unsupported three-level line length");
+};
+
+// Line operator y = F_{-S}(tab . F_S(x)) on three levels.  Element n of
+// line l sits at l*LS + n*ES + (n / N3)*PAD (PAD separates the groups of N3
+// elements that phase C walks).  With n = M n1 + m, m = N3 n2 + n3 and bin
+// k = k1 + N1 (k2 + N2 k3):
+//   A  per (l, m):      N1-point DFT over n1, times W_N^(m k1)
+//   B  per (l, k1, n3): N2-point DFT over n2, times W_M^(n3 k2)
+//   C  per (l, k1, k2): N3-point DFT over n3, table, inverse DFT, times
+//                       W_M^(-n3 k2)
+//   B' per (l, k1, n3): inverse N2-point DFT, times W_N^(-m k1)
+//   A' per (l, m):      inverse N1-point DFT
+// W_M^j is the N-point twiddle W_N^(N1 j).  The same LineSet conventions
+// (fields, tables, task order line-fastest) as line_op_set.
+template <typename T, int N, int S, int LS, int ES, int PAD = 0>
+__device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
+                                             const cplx<T> *__restrict__ tw, int tid, int nthr) {
+    using Q = Split3<N>;
+    constexpr int N1 = Q::N1, N2 = Q::N2, N3 = Q::N3, M = Q::M;
+    auto pos = [](int n) { return n * ES + (n / N3) * PAD; };
+    auto fwd = [&](cplx<T> a, int j) {
+        return S < 0 ? cmul_hl(a, tw[2 * j], tw[2 * j + 1]) : cmulc_hl(a, tw[2 * j], tw[2 * j + 1]);
+    };
+    auto inv = [&](cplx<T> a, int j) {
+        return S < 0 ? cmulc_hl(a, tw[2 * j], tw[2 * j + 1]) : cmul_hl(a, tw[2 * j], tw[2 * j + 1]);
+    };
+    // A: task (l, m, f)
+    for (int task = tid; task < nlines * M * ls.nf; task += nthr) {
+        const int l = task % nlines, r = task / nlines, m = r % M, f = r / M;
+        cplx<T> *base = ls.base(f) + l * LS;
+        cplx<T> v[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[pos(M * n1 + m)];
+        dft<N1, S>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < N1; ++k1) v[k1] = fwd(v[k1], m * k1);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) base[pos(M * k1 + m)] = v[k1];
+    }
+    __syncthreads();
+    // B: task (l, n3, k1, f)
+    for (int task = tid; task < nlines * N3 * N1 * ls.nf; task += nthr) {
+        const int l = task % nlines, r = task / nlines, n3 = r % N3, r2 = r / N3, k1 = r2 % N1, f = r2 / N1;
+        cplx<T> *base = ls.base(f) + l * LS;
+        const int o = M * k1 + n3;
+        cplx<T> v[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = base[pos(o + N3 * n2)];
+        dft<N2, S>(v);
+#pragma unroll
+        for (int k2 = 1; k2 < N2; ++k2) v[k2] = fwd(v[k2], N1 * n3 * k2);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) base[pos(o + N3 * k2)] = v[k2];
+    }
+    __syncthreads();
+    // C: task (l, k2, k1, f)
+    for (int task = tid; task < nlines * N2 * N1 * ls.nf; task += nthr) {
+        const int l = task % nlines, r = task / nlines, k2 = r % N2, r2 = r / N2, k1 = r2 % N1, f = r2 / N1;
+        const cplx<T> *tab = ls.tab(f);
+        cplx<T> *base = ls.base(f) + l * LS;
+        const int o = M * k1 + N3 * k2;
+        cplx<T> v[N3];
+#pragma unroll
+        for (int n3 = 0; n3 < N3; ++n3) v[n3] = base[pos(o + n3)];
+        dft<N3, S>(v);
+#pragma unroll
+        for (int k3 = 0; k3 < N3; ++k3) {
+            const int k = k1 + N1 * (k2 + N2 * k3);
+            v[k3] = cmul_hl(v[k3], tab[2 * k], tab[2 * k + 1]);
+        }
+        dft<N3, -S>(v);
+#pragma unroll
+        for (int n3 = 1; n3 < N3; ++n3) v[n3] = inv(v[n3], N1 * n3 * k2);
+#pragma unroll
+        for (int n3 = 0; n3 < N3; ++n3) base[pos(o + n3)] = v[n3];
+    }
+    __syncthreads();
+    // B': task (l, n3, k1, f)
+    for (int task = tid; task < nlines * N3 * N1 * ls.nf; task += nthr) {
+        const int l = task % nlines, r = task / nlines, n3 = r % N3, r2 = r / N3, k1 = r2 % N1, f = r2 / N1;
+        cplx<T> *base = ls.base(f) + l * LS;
+        const int o = M * k1 + n3;
+        cplx<T> v[N2];
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) v[k2] = base[pos(o + N3 * k2)];
+        dft<N2, -S>(v);
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = inv(v[n2], (N3 * n2 + n3) * k1);
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) base[pos(o + N3 * n2)] = v[n2];
+    }
+    __syncthreads();
+    // A': task (l, m, f)
+    for (int task = tid; task < nlines * M * ls.nf; task += nthr) {
+        const int l = task % nlines, r = task / nlines, m = r % M, f = r / M;
+        cplx<T> *base = ls.base(f) + l * LS;
+        cplx<T> v[N1];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[pos(M * k1 + m)];
+        dft<N1, -S>(v);
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) base[pos(M * n1 + m)] = v[n1];
+    }
+    __syncthreads();
+}
+
+// Lines of 2048 samples take the three-level operator: as N1 * N2 = 64 * 32
+// their first phase has 64-point register DFTs and a quarter of the tasks
+// of the other lengths (measured 1.8x slower per tile); at 512 and 1024 the
+// two-level operator is the faster one (by 3-4 %).
+template <int N> constexpr bool three_level() { return N == 2048; }
+// Length of the innermost group of a line (the groups that padding
+// separates in LDS).
+template <int N> constexpr int inner_len() {
+    if constexpr (three_level<N>()) return Split3<N>::N3;
+    else return Split<N>::N2;
+}
+
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false>
+__device__ __forceinline__ void line_op_any(const LineSet<T> &ls, int nlines, const cplx<T> *__restrict__ tw,
+                                            int tid, int nthr) {
+    if constexpr (three_level<N>()) line_op_set3<T, N, S, LS, ES, PAD>(ls, nlines, tw, tid, nthr);
+    else line_op_set<T, N, S, LS, ES, PAD, UNI>(ls, nlines, tw, tid, nthr);
+}
+
 // One field.
 template <typename T, int N, int S, int LS, int ES, bool UNI = false>
 __device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> *__restrict__ tw,
                                         const cplx<T> *__restrict__ tab, int tid, int nthr) {
     LineSet<T> ls{lds, 0, tab, 0, 1};
-    line_op_set<T, N, S, LS, ES, 0, UNI>(ls, nlines, tw, tid, nthr);
+    line_op_any<T, N, S, LS, ES, 0, UNI>(ls, nlines, tw, tid, nthr);
 }
 
 }  // namespace opo

@@ -29,13 +29,15 @@ template <typename T, int NT> constexpr int tline3_tx() {
 }
 
 // LDS layout of one field's tile in the fused pass: time row t (TX
-// pixels) starts at t*TX + (t / N2)*PAD.  Narrow tiles pad every N2 rows by
-// TX elements so the N1 rows of the second phase fall in different banks.
+// pixels) starts at t*TX + (t / G)*PAD, G the innermost group length of
+// the line operator (N2 of the two-level split, N3 of the three-level one).
+// Narrow tiles pad every G rows by TX elements so the rows that the
+// innermost phase walks in parallel fall in different banks.
 template <typename T, int NT, int TX> struct TimeTile {
-    static constexpr int N1 = Split<NT>::N1, N2 = Split<NT>::N2;
-    static constexpr int PAD = (N2 > 1 && TX < 16) ? TX : 0;
-    static constexpr int FT = NT * TX + N1 * PAD;   // elements per field
-    __device__ static __forceinline__ int at(int i) { return i + (i / (N2 * TX)) * PAD; }
+    static constexpr int G = inner_len<NT>();
+    static constexpr int PAD = (G > 1 && TX < 16) ? TX : 0;
+    static constexpr int FT = NT * TX + (NT / G) * PAD;   // elements per field
+    __device__ static __forceinline__ int at(int i) { return i + (i / (G * TX)) * PAD; }
 };
 
 // Launch variants of the fused pass for one NT (chosen by the autotuner).
@@ -185,7 +187,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     }
     __syncthreads();
     const LineSet<T> ls{lds + a.first_field * FT, FT, a.tab0, a.tab_stride, 3 - a.first_field};
-    line_op_set<T, NT, +1, 1, TX, L::PAD>(ls, TX, tw, tid, BS);
+    line_op_any<T, NT, +1, 1, TX, L::PAD>(ls, TX, tw, tid, BS);
     if constexpr (TX % 2 == 0) {
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const int t = i / TX, xl = i % TX;
@@ -283,7 +285,7 @@ __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T
         __syncthreads();
         if (u + 1 < u1) load(u + 1);
         const LineSet<T> ls{lds + a.first_field * FT, FT, stab + 2 * NT, 2 * NT, 3 - a.first_field};
-        line_op_set<T, NT, +1, 1, TX, L::PAD>(ls, TX, stab, tid, BS);
+        line_op_any<T, NT, +1, 1, TX, L::PAD>(ls, TX, stab, tid, BS);
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const size_t gi = offset(u, i);
             const int li = L::at(i);
