@@ -154,22 +154,23 @@ __global__ void __launch_bounds__(BS) kb_xyp(const SolveDesc<T> *__restrict__ sd
     xy_run<T, NX, BS>(XyBatch<T, NX>{sd, ts, status, work, n_work, free_space}, lds, stab);
 }
 
-template <typename T, int NX, int BS>
+template <typename T, int NX, int BS, int P>
 __global__ void __launch_bounds__(BS) kb_xy(const SolveDesc<T> *__restrict__ sd,
                                             const TripState *__restrict__ ts,
                                             const int *__restrict__ status,
                                             const WorkItem *__restrict__ work, int free_space) {
-    __shared__ cplx<T> lds[NX * (NX + 1)];
+    __shared__ cplx<T> lds[P * NX * (NX + 1)];
     const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
     if (status[w.solve]) return;
     if (free_space && !ts[w.solve].fs) return;
     const SolveDesc<T> &d = sd[w.solve];
-    const int k = w.item / d.nt, t = w.item % d.nt;
+    const int groups = d.nt / P;   // item = k * groups + group of P planes
+    const int k = w.item / groups, t = (w.item % groups) * P;
     const int f = free_space ? 1 + k : k;
     const cplx<T> *tx = free_space ? d.qfsx + 2 * NX * k : d.qx + 2 * NX * k;
     const cplx<T> *ty = free_space ? d.qfsy + 2 * NX * k : d.qy + 2 * NX * k;
     cplx<T> *A = (f == 0 ? d.f[0] : (f == 1 ? d.f[1] : d.f[2])) + (size_t)t * NX * NX;
-    xy_plane<T, NX, BS>(A, tx, ty, d.twx, lds);
+    xy_plane<T, NX, BS, P>(A, tx, ty, d.twx, lds);
 }
 
 __device__ __forceinline__ void neumaier(double &sum, double &comp, double v) {
@@ -284,30 +285,42 @@ hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc
     }
 }
 
+template <typename T, int NX, int P>
+static hipError_t bx_launch_bs(int bs, hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
+                               const int *status, const WorkItem *work, int fs) {
+    const dim3 g((unsigned)n_work);
+    if (bs == 256) hipLaunchKernelGGL((kb_xy<T, NX, 256, P>), g, dim3(256), 0, st, sd, ts, status, work, fs);
+    else if (bs == 512) hipLaunchKernelGGL((kb_xy<T, NX, 512, P>), g, dim3(512), 0, st, sd, ts, status, work, fs);
+    else hipLaunchKernelGGL((kb_xy<T, NX, 1024, P>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
+    return hipGetLastError();
+}
+
 template <typename T, int NX>
 static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
                               const int *status, const WorkItem *work, int fs) {
-    const dim3 g((unsigned)n_work);
     if constexpr (sizeof(T) == 8) {
-        constexpr int BS = NX >= 96 ? 512 : 256;
-        hipLaunchKernelGGL((kb_xy<T, NX, BS>), g, dim3(BS), 0, st, sd, ts, status, work, fs);
+        constexpr int v = xy_default_variant(NX);
+        constexpr int BS = xy_variant_bs(NX, v);
+        hipLaunchKernelGGL((kb_xy<T, NX, BS, xy_group(NX, v)>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts,
+                           status, work, fs);
+        return hipGetLastError();
     } else {
         const int v = tune_get(false, 1, NX);
-        const int bs = xy_variant_bs(v);
-        if (xy_variant_persistent(v)) {
-            const dim3 pg(persistent_grid(n_work, xy_variant_per_cu(v)));
-            if (bs == 256) hipLaunchKernelGGL((kb_xyp<T, NX, 256>), pg, dim3(256), 0, st, sd, ts, status, work, n_work, fs);
-            else if (bs == 512) hipLaunchKernelGGL((kb_xyp<T, NX, 512>), pg, dim3(512), 0, st, sd, ts, status, work, n_work, fs);
-            else hipLaunchKernelGGL((kb_xyp<T, NX, 1024>), pg, dim3(1024), 0, st, sd, ts, status, work, n_work, fs);
-        } else if (bs == 256) {
-            hipLaunchKernelGGL((kb_xy<T, NX, 256>), g, dim3(256), 0, st, sd, ts, status, work, fs);
-        } else if (bs == 512) {
-            hipLaunchKernelGGL((kb_xy<T, NX, 512>), g, dim3(512), 0, st, sd, ts, status, work, fs);
+        const int bs = xy_variant_bs(NX, v);
+        if constexpr (NX >= 64) {
+            if (xy_variant_persistent(NX, v)) {
+                const dim3 pg(persistent_grid(n_work, xy_variant_per_cu(NX, v)));
+                if (bs == 256) hipLaunchKernelGGL((kb_xyp<T, NX, 256>), pg, dim3(256), 0, st, sd, ts, status, work, n_work, fs);
+                else if (bs == 512) hipLaunchKernelGGL((kb_xyp<T, NX, 512>), pg, dim3(512), 0, st, sd, ts, status, work, n_work, fs);
+                else hipLaunchKernelGGL((kb_xyp<T, NX, 1024>), pg, dim3(1024), 0, st, sd, ts, status, work, n_work, fs);
+                return hipGetLastError();
+            }
+            return bx_launch_bs<T, NX, 1>(bs, st, n_work, sd, ts, status, work, fs);
         } else {
-            hipLaunchKernelGGL((kb_xy<T, NX, 1024>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
+            if (xy_group(NX, v) > 1) return bx_launch_bs<T, NX, xy_group(NX, 3)>(bs, st, n_work, sd, ts, status, work, fs);
+            return bx_launch_bs<T, NX, 1>(bs, st, n_work, sd, ts, status, work, fs);
         }
     }
-    return hipGetLastError();
 }
 
 template <typename T>

@@ -692,6 +692,8 @@ template <typename T> struct SolveT final : SolveImpl {
                 const int pc = tune_per_cu(fp64, axis, n, v);
                 char pers[32] = "";
                 if (pc) std::snprintf(pers, sizeof(pers), ", persistent %d/CU", pc);
+                else if (axis == 1 && xy_group(n, v) > 1)
+                    std::snprintf(pers, sizeof(pers), ", %d planes", xy_group(n, v));
                 std::snprintf(buf, sizeof(buf), " v%d(tile %d, block %d%s) %.2f;", v,
                               axis == 0 ? tune_tile_px(fp64, n, v) : n, tune_block(fp64, axis, n, v), pers, us);
                 report += buf;
@@ -937,15 +939,21 @@ struct opo_batch {
             trange.push_back(r);
         }
         for (auto &kv : by_nx) {
+            // one item per group of consecutive planes of a field (xy_group)
+            const int pg = xy_group_tuned(fp64, kv.first);
             Range r{kv.first, (int)work.size(), 0};
-            for (int i : kv.second)
-                for (int it = 0; it < 3 * resident[i]->g.nt; ++it) work.push_back(WorkItem{i, it});
+            for (int i : kv.second) {
+                if (resident[i]->g.nt % pg != 0)
+                    return fail(OPO_ERR_UNSUPPORTED, "NT=%d is not a multiple of the %d-plane xy group (NX=%d)",
+                                resident[i]->g.nt, pg, kv.first);
+                for (int it = 0; it < 3 * resident[i]->g.nt / pg; ++it) work.push_back(WorkItem{i, it});
+            }
             r.n = (int)work.size() - r.off;
             xyrange.push_back(r);
             Range q{kv.first, (int)work.size(), 0};
             for (int i : kv.second)
                 if (resident[i]->p.fs_on)
-                    for (int it = 0; it < 2 * resident[i]->g.nt; ++it) work.push_back(WorkItem{i, it});
+                    for (int it = 0; it < 2 * resident[i]->g.nt / pg; ++it) work.push_back(WorkItem{i, it});
             q.n = (int)work.size() - q.off;
             if (q.n) fsrange.push_back(q);
         }

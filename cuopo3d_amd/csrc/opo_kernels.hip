@@ -246,7 +246,8 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
     if (fp64) return v == xy_default_variant(n);
     if (n < 16) return v == 0;
-    if (xy_variant_persistent(v)) return n * n / xy_variant_bs(v) <= 16;
+    if (n <= 32) return v < 6;
+    if (xy_variant_persistent(n, v)) return n * n / xy_variant_bs(n, v) <= 16;
     return true;
 }
 
@@ -275,14 +276,14 @@ int tune_tile_px(bool fp64, int nt, int v) {
 }
 
 int tune_per_cu(bool fp64, int axis, int n, int v) {
-    if (axis == 1) return xy_variant_per_cu(v);
+    if (axis == 1) return xy_variant_per_cu(n, v);
     int tx = 0, bs = 0, per_cu = 0;
     const bool ok = fp64 ? time_info<double>(n, v, tx, bs, per_cu) : time_info<float>(n, v, tx, bs, per_cu);
     return ok ? per_cu : 0;
 }
 
 int tune_block(bool fp64, int axis, int n, int v) {
-    if (axis == 1) return xy_variant_bs(v);
+    if (axis == 1) return xy_variant_bs(n, v);
     int tx = 0, bs = 0;
     const bool ok = fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
     return ok ? bs : 0;
@@ -356,13 +357,13 @@ hipError_t launch_tline3(hipStream_t st, const Geometry &g, const TLine3Args<T> 
 }
 
 // --------------------------------------------------------- xy operator
-template <typename T, int NX, int BS>
+template <typename T, int NX, int BS, int P>
 __global__ void __launch_bounds__(BS) k_xyplane(FieldOps<T> ops, int nt, const cplx<T> *__restrict__ tw,
                                                 const int *status) {
     if (status && *status) return;
-    __shared__ cplx<T> lds[NX * (NX + 1)];
-    const int t = blockIdx.x, f = blockIdx.y;
-    xy_plane<T, NX, BS>(ops.f[f] + (size_t)t * NX * NX, ops.tab[f], ops.taby[f], tw, lds);
+    __shared__ cplx<T> lds[P * NX * (NX + 1)];
+    const int t = blockIdx.x * P, f = blockIdx.y;
+    xy_plane<T, NX, BS, P>(ops.f[f] + (size_t)t * NX * NX, ops.tab[f], ops.taby[f], tw, lds);
 }
 
 template <typename T, int NX, int BS>
@@ -374,30 +375,53 @@ __global__ void __launch_bounds__(BS) k_xyplanep(FieldOps<T> ops, int nt, int nf
     xy_run<T, NX, BS>(XySolo<T, NX>{ops, nt, nf, tw}, lds, stab);
 }
 
+template <typename T, int NX, int BS, int P>
+static hipError_t xy_launch_groups(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
+                                   const cplx<T> *tw, const int *status) {
+    if (g.nt % P != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_xyplane<T, NX, BS, P>), dim3((unsigned)(g.nt / P), (unsigned)nf), dim3(BS), 0, st, ops,
+                       g.nt, tw, status);
+    return hipGetLastError();
+}
+
+template <typename T, int NX, int P>
+static hipError_t xy_launch_bs(int bs, hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
+                               const cplx<T> *tw, const int *status) {
+    if (bs == 256) return xy_launch_groups<T, NX, 256, P>(st, g, ops, nf, tw, status);
+    if (bs == 512) return xy_launch_groups<T, NX, 512, P>(st, g, ops, nf, tw, status);
+    return xy_launch_groups<T, NX, 1024, P>(st, g, ops, nf, tw, status);
+}
+
 template <typename T, int NX>
 static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                               const cplx<T> *tw, const int *status) {
-    dim3 grid((unsigned)g.nt, (unsigned)nf);
-    const int v = tune_get(sizeof(T) == 8, 1, NX);
-    if constexpr (NX < 16 || sizeof(T) == 8) {
-        constexpr int BS = NX >= 96 ? 512 : 256;
-        hipLaunchKernelGGL((k_xyplane<T, NX, BS>), grid, dim3(BS), 0, st, ops, g.nt, tw, status);
+    if constexpr (NX < 16) {
+        return xy_launch_groups<T, NX, 256, 1>(st, g, ops, nf, tw, status);
+    } else if constexpr (sizeof(T) == 8) {
+        constexpr int v = xy_default_variant(NX);
+        return xy_launch_groups<T, NX, xy_variant_bs(NX, v), xy_group(NX, v)>(st, g, ops, nf, tw, status);
     } else {
-        const int bs = xy_variant_bs(v);
-        if (xy_variant_persistent(v)) {
-            const dim3 pg(persistent_grid(g.nt * nf, xy_variant_per_cu(v)));
-            if (bs == 256) hipLaunchKernelGGL((k_xyplanep<T, NX, 256>), pg, dim3(256), 0, st, ops, g.nt, nf, tw, status);
-            else if (bs == 512) hipLaunchKernelGGL((k_xyplanep<T, NX, 512>), pg, dim3(512), 0, st, ops, g.nt, nf, tw, status);
-            else hipLaunchKernelGGL((k_xyplanep<T, NX, 1024>), pg, dim3(1024), 0, st, ops, g.nt, nf, tw, status);
-        } else if (bs == 256) {
-            hipLaunchKernelGGL((k_xyplane<T, NX, 256>), grid, dim3(256), 0, st, ops, g.nt, tw, status);
-        } else if (bs == 512) {
-            hipLaunchKernelGGL((k_xyplane<T, NX, 512>), grid, dim3(512), 0, st, ops, g.nt, tw, status);
+        const int v = tune_get(false, 1, NX);
+        const int bs = xy_variant_bs(NX, v);
+        if constexpr (NX >= 64) {
+            if (xy_variant_persistent(NX, v)) {
+                const dim3 pg(persistent_grid(g.nt * nf, xy_variant_per_cu(NX, v)));
+                if (bs == 256) hipLaunchKernelGGL((k_xyplanep<T, NX, 256>), pg, dim3(256), 0, st, ops, g.nt, nf, tw, status);
+                else if (bs == 512) hipLaunchKernelGGL((k_xyplanep<T, NX, 512>), pg, dim3(512), 0, st, ops, g.nt, nf, tw, status);
+                else hipLaunchKernelGGL((k_xyplanep<T, NX, 1024>), pg, dim3(1024), 0, st, ops, g.nt, nf, tw, status);
+                return hipGetLastError();
+            }
+            return xy_launch_bs<T, NX, 1>(bs, st, g, ops, nf, tw, status);
         } else {
-            hipLaunchKernelGGL((k_xyplane<T, NX, 1024>), grid, dim3(1024), 0, st, ops, g.nt, tw, status);
+            if (xy_group(NX, v) > 1) return xy_launch_bs<T, NX, xy_group(NX, 3)>(bs, st, g, ops, nf, tw, status);
+            return xy_launch_bs<T, NX, 1>(bs, st, g, ops, nf, tw, status);
         }
     }
-    return hipGetLastError();
+}
+
+int xy_group_tuned(bool fp64, int nx) {
+    if (nx < 16) return 1;
+    return xy_group(nx, fp64 ? xy_default_variant(nx) : tune_get(false, 1, nx));
 }
 
 bool xyplane_supported(int nx, int ny, bool fp64) {

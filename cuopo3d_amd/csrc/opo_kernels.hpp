@@ -97,6 +97,18 @@ int tune_per_cu(bool fp64, int axis, int n, int v);
 // device, at most one per unit.
 unsigned persistent_grid(int units, int per_cu);  // persistent workgroups per CU, 0: one unit each
 
+// Consecutive t-planes of one field that one workgroup of xy variant v
+// transforms.  Lines of at most 32 points are single register DFTs, one
+// task per line, so one 32 x 32 plane gives a 256-thread workgroup 32
+// tasks per phase; variants 3-5 of NX <= 32 take 256 / NX planes (256
+// tasks), the others one plane.  Every supported NT (16 and up) is a
+// multiple of the group.
+__host__ __device__ constexpr int xy_group(int nx, int v) {
+    return (nx > 1 && nx <= 32 && v >= 3 && v < 6) ? 256 / nx : 1;
+}
+// Plane group of the xy variant currently chosen for NX (host side).
+int xy_group_tuned(bool fp64, int nx);
+
 bool tline_supported(int nt, bool fp64);
 bool xyplane_supported(int nx, int ny, bool fp64);
 

@@ -74,19 +74,22 @@ template <typename T, int NT, int V> struct TimeVariant {
 constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 64) ? 1 : 0; }
 
 // Variants of the xy pass: V < 3 one plane per workgroup of 256, 512 or
-// 1024 threads; V >= 3 persistent workgroups (k + 1 per CU) walking a run
-// of planes with the next plane prefetched into registers,
-// V - 3 = 2 b + k with the same block sizes.  Persistent variants need at
-// most 16 prefetched elements per thread.  (Not offered, measured no
-// faster: all three fields of one time sample per workgroup; 512 threads
-// held to 80 registers so that three 64 x 64 planes share a CU.)
+// 1024 threads; for NX >= 64, V >= 3 persistent workgroups (k + 1 per CU)
+// walking a run of planes with the next plane prefetched into registers,
+// V - 3 = 2 b + k with the same block sizes (at most 16 prefetched
+// elements per thread); for 16 <= NX <= 32, V = 3 + b groups of 256 / NX
+// planes per workgroup (xy_group) with the same block sizes.  (Not
+// offered, measured no faster: all three fields of one time sample per
+// workgroup; 512 threads held to 80 registers so that three 64 x 64
+// planes share a CU; one line per thread on 64-256 threads for NX <= 32;
+// persistent workgroups for NX <= 32.)
 constexpr int kXyVariants = 9;
-constexpr int xy_variant_bs(int v) {
-    const int b = v < 3 ? v : (v - 3) / 2;
+constexpr bool xy_variant_persistent(int nx, int v) { return nx >= 64 && v >= 3; }
+constexpr int xy_variant_bs(int nx, int v) {
+    const int b = xy_variant_persistent(nx, v) ? (v - 3) / 2 : v % 3;
     return b == 0 ? 256 : (b == 1 ? 512 : 1024);
 }
-constexpr bool xy_variant_persistent(int v) { return v >= 3; }
-constexpr int xy_variant_per_cu(int v) { return v >= 3 ? (v - 3) % 2 + 1 : 0; }
+constexpr int xy_variant_per_cu(int nx, int v) { return xy_variant_persistent(nx, v) ? (v - 3) % 2 + 1 : 0; }
 constexpr int xy_default_variant(int nx) { return nx >= 96 ? 1 : 0; }
 
 template <typename T> struct alignas(4 * sizeof(T)) cpair {
@@ -328,23 +331,27 @@ template <typename T, int NT, int TX> struct TimeSolo {
 // constants wave-uniformly (through the scalar cache).
 template <int NX, int BS> constexpr bool xy_uniform() { return NX % 64 == 0 && BS % 64 == 0; }
 
-// Q_x then Q_y on the NX x NX plane A (x fastest).  lds holds NX*(NX+1)
+// Q_x then Q_y on P consecutive NX x NX planes starting at A (x fastest,
+// planes NX*NX apart).  lds holds P*NX*(NX+1)
 // elements: rows padded by one element so row and column walks spread over
-// the LDS banks.
-template <typename T, int NX, int BS>
+// the LDS banks.  The planes share the tables (one field); each line goes
+// through exactly the operations it would alone.
+template <typename T, int NX, int BS, int P = 1>
 __device__ __forceinline__ void xy_plane(cplx<T> *A, const cplx<T> *__restrict__ tabx,
                                          const cplx<T> *__restrict__ taby,
                                          const cplx<T> *__restrict__ tw, cplx<T> *lds) {
-    constexpr int NXP = NX + 1;
+    constexpr int NXP = NX + 1, NE = P * NX * NX;
     const int tid = threadIdx.x;
-    for (int i = tid; i < NX * NX; i += BS) lds[(i / NX) * NXP + (i % NX)] = A[i];
+    for (int i = tid; i < NE; i += BS) lds[(i / NX) * NXP + (i % NX)] = A[i];
     __syncthreads();
-    // Q_x: lines are rows (pitch NXP), elements contiguous.
     constexpr bool UNI = xy_uniform<NX, BS>();
-    line_op<T, NX, -1, NXP, 1, UNI>(lds, NX, tw, tabx, tid, BS);
+    // Q_x: lines are rows (pitch NXP), elements contiguous.
+    const LineSet<T> lx{lds, NX * NXP, tabx, 0, P};
+    line_op_any<T, NX, -1, NXP, 1, 0, UNI>(lx, NX, tw, tid, BS);
     // Q_y: lines are columns, elements one row apart.
-    line_op<T, NX, -1, 1, NXP, UNI>(lds, NX, tw, taby, tid, BS);
-    for (int i = tid; i < NX * NX; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
+    const LineSet<T> ly{lds, NX * NXP, taby, 0, P};
+    line_op_any<T, NX, -1, 1, NXP, 0, UNI>(ly, NX, tw, tid, BS);
+    for (int i = tid; i < NE; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
 }
 
 // Persistent form of the xy pass: the workgroup walks a run of planes and
