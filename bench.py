@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--grid", type=int, nargs=2, default=(64, 256), metavar=("NXY", "NT"),
                     help="grid of the cavity workload (default 64 256); other grids are for "
                          "profiling and skip the CPU and comparator legs")
+    ap.add_argument("--no-reference-grid", action="store_true",
+                    help="skip the run on the reference's compiled-default 128x128x2048 grid")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 128 MiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
@@ -143,6 +145,8 @@ def main():
     if default_grid:
         out["idler_dumped"] = idler_dumped_leg(co, configs, args.steps)
         out["fp64_path"] = fp64_leg(co, configs, min(args.steps, 50))
+        if not args.no_reference_grid:
+            out["reference_default_grid"] = reference_grid_leg(co, configs, args.tune_reps)
     if not args.no_cpu:
         from oracle.pyoracle import Oracle, get_threads
         g2, p2, _, f2 = configs.cavity_64(n_round_trips=args.cpu_trips)
@@ -214,6 +218,26 @@ def horizon_leg(co, configs, trips):
     s32.close()
     s64.close()
     return res
+
+
+def reference_grid_leg(co, configs, tune_reps, trips=3):
+    """The same cavity on the grid the reference compiles by default
+    (NX = NY = 128, NT = 2048: DtypesConsts.cuh:30-54), tuned for it."""
+    g, p, _, f = configs.cavity_64(n=(128, 128, 2048), n_round_trips=trips + 1)
+    s = co.Solver(g, p, device=0)
+    s.set_fields(*f)
+    report = s.autotune(reps=max(1, min(tune_reps, 2)))
+    s.runOPO(1)
+    s.sync()
+    t0 = time.perf_counter()
+    s.runOPO(trips)
+    s.sync()
+    dt = time.perf_counter() - t0
+    s.close()
+    cells = g.nx * g.ny * g.nt
+    return {"grid": [g.nx, g.ny, g.nt], "trips": trips, "value": cells * p.n_slices * trips / dt,
+            "unit": "cell-steps/s", "ms_per_step": dt / trips * 1e3,
+            "autotune": [ln[-12:] for ln in report.strip().splitlines()]}
 
 
 def fp64_leg(co, configs, trips):

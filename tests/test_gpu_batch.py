@@ -62,6 +62,39 @@ def test_batch_equals_single_solve_and_oracle(require_gpu):
         assert np.max(np.abs(pb - pr) / np.abs(pr)) < 1e-6
 
 
+def test_batch_long_and_grouped_lines_equal_single_solves(require_gpu):
+    """A 2048-sample solve (three-level t operator) and 16- and 32-pixel
+    solves (xy groups of planes when tuned so) batched together give the
+    single-solve fields bitwise, with the small-plane xy variant set to a
+    group of planes and to one plane."""
+    from cuopo3d_amd.capi import OPO_FP32, OPO_TUNE_XY_PASS, OPO_OK
+    lib = co.load()
+    draws = configs.sweep_draws(n_solves=4, seed=5)
+    solves = []
+    for d, (nxy, nt) in zip(draws, [(16, 2048), (32, 128), (16, 256)]):
+        d = dict(d)
+        d["nxy"], d["nt"] = nxy, nt
+        solves.append(configs.sweep_solve(d, n_round_trips=2))
+    keep = {nx: lib.opo_tune_get(OPO_FP32, OPO_TUNE_XY_PASS, nx) for nx in (16, 32)}
+    try:
+        for v in (4, 0):   # 512 threads on 256/NX planes, then one plane of 256
+            for nx in (16, 32):
+                assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, nx, v) == OPO_OK
+            mixed, _ = run_and_collect(solves, 2)
+            for (g, p, ph, f), (pb, fb) in zip(solves, mixed):
+                s = co.Solver(g, p, device=0)
+                s.set_fields(*f)
+                ps = s.runOPO(2)
+                for a, b in zip(s.get_fields(), fb):
+                    assert np.array_equal(a, b), (v, g.nx, g.nt)
+                assert np.allclose(ps, pb, rtol=1e-13, atol=0)
+                s.close()
+            print("xy variant", v, ": batch == single for", [(s[0].nx, s[0].nt) for s in solves], flush=True)
+    finally:
+        for nx, v in keep.items():
+            lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, nx, v)
+
+
 def test_batch_admission_waits_for_budget(require_gpu):
     draws = configs.sweep_draws(n_solves=3, seed=9)
     for d in draws:
