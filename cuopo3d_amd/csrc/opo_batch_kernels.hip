@@ -41,6 +41,7 @@ __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ s
                                               const int *__restrict__ status,
                                               const WorkItem *__restrict__ work, int mode) {
     __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
+    __shared__ cplx<T> twl[time_stage_twiddles<T, NT, TX>() ? 2 * NT : 1];
     const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
     if (status[w.solve]) return;
     const SolveDesc<T> &d = sd[w.solve];
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ s
         a.first_field = 0;
     }
     a.rk = batch_rk(d, mode);
-    nl_time_tile<T, NT, TX, BS, false>(a, d.nx, d.ny, w.item, d.twt, lds);
+    nl_time_tile<T, NT, TX, BS, false>(a, d.nx, d.ny, w.item, d.twt, lds, twl);
 }
 
 // Work units of the fused pass over a work list: tile w.item of solve w.solve.

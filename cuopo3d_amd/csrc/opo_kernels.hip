@@ -139,7 +139,8 @@ __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
                                                const cplx<T> *__restrict__ tw, const int *status) {
     if (status && *status) return;
     __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
-    nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds);
+    __shared__ cplx<T> twl[time_stage_twiddles<T, NT, TX>() ? 2 * NT : 1];
+    nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds, twl);
 }
 
 template <typename T, int NT, int TX, int BS>

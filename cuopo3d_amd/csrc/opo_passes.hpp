@@ -142,14 +142,37 @@ __device__ __forceinline__ void nl_point(const Rk4Args<T> &rk, int t, Tri<T> &A)
     }
 }
 
+// Whether a one-tile workgroup of the fused pass copies the NT-point
+// twiddles (2 NT elements, hi/lo) into LDS: the line operator then reads
+// them from LDS instead of every lane fetching its (hi, lo) pairs through
+// the vector memory path, which for 32- and 16-point register DFTs is the
+// larger part of a phase (128 x 128 x 1024 t pass 462 -> 432 us).  Only for
+// the two-level lines of 512 and 1024 samples and where the copy fits
+// beside the tile (the three-level 2048-sample lines measured 4 % slower
+// with it).
+template <typename T, int NT, int TX> constexpr bool time_stage_twiddles() {
+    return (NT == 512 || NT == 1024) &&
+           (3 * TimeTile<T, NT, TX>::FT + 2 * NT) * (int)sizeof(cplx<T>) <= 150 * 1024;
+}
+
 // One tile of the fused pass: TX consecutive pixels of row y, all NT time
-// samples, all three fields.  lds holds 3 * NT * TX elements.
+// samples, all three fields.  lds holds 3 * NT * TX elements; twl holds the
+// 2 NT staged twiddles where time_stage_twiddles() (else unused).
 template <typename T, int NT, int TX, int BS, bool PHASE>
 __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int ny, int tile,
-                                             const cplx<T> *__restrict__ tw, cplx<T> *lds) {
+                                             const cplx<T> *__restrict__ tw, cplx<T> *lds, cplx<T> *twl) {
     using L = TimeTile<T, NT, TX>;
     constexpr int TILE = NT * TX, FT = L::FT;
+    constexpr bool STW = time_stage_twiddles<T, NT, TX>();
+    constexpr int TWP = STW ? (2 * NT + BS - 1) / BS : 1;
     const int tid = threadIdx.x;
+    // twiddle copy: loads issued now, stored to LDS after the pointwise step
+    cplx<T> twr[TWP];
+    if constexpr (STW) {
+#pragma unroll
+        for (int j = 0; j < TWP; ++j)
+            if (tid + j * BS < 2 * NT) twr[j] = tw[tid + j * BS];
+    }
     const int tiles_x = nx / TX;
     const int y = tile / tiles_x, x0 = (tile % tiles_x) * TX;
     const size_t plane = (size_t)nx * ny;
@@ -188,9 +211,14 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             lds[2 * FT + li] = A.i;
         }
     }
+    if constexpr (STW) {
+#pragma unroll
+        for (int j = 0; j < TWP; ++j)
+            if (tid + j * BS < 2 * NT) twl[tid + j * BS] = twr[j];
+    }
     __syncthreads();
     const LineSet<T> ls{lds + a.first_field * FT, FT, a.tab0, a.tab_stride, 3 - a.first_field};
-    line_op_any<T, NT, +1, 1, TX, L::PAD>(ls, TX, tw, tid, BS);
+    line_op_any<T, NT, +1, 1, TX, L::PAD>(ls, TX, STW ? twl : tw, tid, BS);
     if constexpr (TX % 2 == 0) {
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const int t = i / TX, xl = i % TX;
