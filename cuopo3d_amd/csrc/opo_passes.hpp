@@ -146,12 +146,12 @@ __device__ __forceinline__ void nl_point(const Rk4Args<T> &rk, int t, Tri<T> &A)
 // twiddles (2 NT elements, hi/lo) into LDS: the line operator then reads
 // them from LDS instead of every lane fetching its (hi, lo) pairs through
 // the vector memory path, which for 32- and 16-point register DFTs is the
-// larger part of a phase (128 x 128 x 1024 t pass 462 -> 432 us).  Only for
-// the two-level lines of 512 and 1024 samples and where the copy fits
-// beside the tile (the three-level 2048-sample lines measured 4 % slower
-// with it).
+// larger part of a phase (128 x 128 x 1024 t pass 462 -> 432 us; cavity
+// 64 x 64 x 256 +1.2 %, 32 x 32 x 128 +2.5 %).  Only for the two-level lines
+// of 128 to 1024 samples and where the copy fits beside the tile (64-sample
+// lines measured no faster, the three-level 2048-sample lines 4 % slower).
 template <typename T, int NT, int TX> constexpr bool time_stage_twiddles() {
-    return (NT == 512 || NT == 1024) &&
+    return NT >= 128 && NT <= 1024 &&
            (3 * TimeTile<T, NT, TX>::FT + 2 * NT) * (int)sizeof(cplx<T>) <= 150 * 1024;
 }
 
