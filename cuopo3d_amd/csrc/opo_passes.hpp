@@ -149,7 +149,8 @@ __device__ __forceinline__ void nl_point(const Rk4Args<T> &rk, int t, Tri<T> &A)
 // larger part of a phase (128 x 128 x 1024 t pass 462 -> 432 us; cavity
 // 64 x 64 x 256 +1.2 %, 32 x 32 x 128 +2.5 %).  Only for the two-level lines
 // of 128 to 1024 samples and where the copy fits beside the tile (64-sample
-// lines measured no faster, the three-level 2048-sample lines 4 % slower).
+// lines measured no faster, the three-level 2048-sample lines 4 % slower;
+// copying the line tables as well measured 0.1-1.3 % slower).
 template <typename T, int NT, int TX> constexpr bool time_stage_twiddles() {
     return NT >= 128 && NT <= 1024 &&
            (3 * TimeTile<T, NT, TX>::FT + 2 * NT) * (int)sizeof(cplx<T>) <= 150 * 1024;
