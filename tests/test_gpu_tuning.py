@@ -100,7 +100,9 @@ def test_autotune_keeps_fields_and_picks_a_variant(restore_tuning, require_gpu):
 
 
 def _persistent(lib, axis, n):
-    """Valid persistent variants of size n (time: V >= 9, xy: V >= 3)."""
+    """Valid variants of size n past the one-unit-per-workgroup ones (time:
+    V >= 9, persistent; xy: V >= 3, persistent for NX >= 64, plane groups
+    for NX <= 32)."""
     first = 9 if axis == OPO_TUNE_TIME_PASS else 3
     return [v for v in _valid(lib, axis, n) if v >= first]
 
