@@ -148,6 +148,7 @@ template <typename T> struct SolveT final : SolveImpl {
     int *flags = nullptr;     // [0] status, [1] trip counter
     int nblk = 64;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_variants[2] = {-1, -1};   // (t, xy) launch variants the graphs were captured with
 
     ~SolveT() override {
         (void)hipSetDevice(device);
@@ -559,6 +560,15 @@ template <typename T> struct SolveT final : SolveImpl {
     int launch_trip(bool fs) {
         int v = fs ? 0 : 1;
         if (!use_graphs) return q_trip(fs);
+        // the launch variants are process-wide: re-capture when they changed
+        const bool fp64 = sizeof(T) == 8;
+        const int cur[2] = {tune_get(fp64, 0, g.nt), tune_get(fp64, 1, g.nx)};
+        if (cur[0] != graph_variants[0] || cur[1] != graph_variants[1]) {
+            for (auto &gx : graph)
+                if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
+            graph_variants[0] = cur[0];
+            graph_variants[1] = cur[1];
+        }
         if (!graph[v]) {
             hipGraph_t gr = nullptr;
             CK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));

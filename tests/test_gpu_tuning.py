@@ -135,3 +135,24 @@ def test_batch_with_persistent_variants(restore_tuning, require_gpu):
                 assert np.array_equal(a, b), (pick, k)
             assert np.array_equal(pr, pg), (pick, k)
         print("persistent batch == classic batch, pick", pick, flush=True)
+
+
+def test_trip_graph_follows_tuning_changes(restore_tuning, require_gpu):
+    """A solve's captured trip graph is re-captured when the process-wide
+    variants change between trips (here to and from a plane-group xy
+    variant, which changes the launch grid); the fields stay bitwise those
+    of a run on one variant."""
+    lib = restore_tuning
+    grid, p, _, fields = configs.cavity_64(n=(32, 32, 128), n_round_trips=3)
+    lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 32, 0)
+    ref, pref = _run(grid, p, fields, trips=3)
+    with co.Solver(grid, p, device=0) as s:
+        s.set_fields(*fields)
+        pw = [s.runOPO(1)]
+        for v in (4, 0):   # 512 threads on 8 planes, then one plane again
+            assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 32, v) == OPO_OK
+            pw.append(s.runOPO(1))
+        got = s.get_fields()
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    assert np.array_equal(np.concatenate(pw), pref)
