@@ -49,10 +49,14 @@ template <typename T, int NT, int TX> struct TimeTile {
 // V - 9 = 2 (3 m + b) + k with b = 0: 256, 1: 384, 2: 512 threads.
 // Only variant 0 exists for FP64 and for NT < 64; a variant whose three
 // tiles exceed 112 KiB of LDS (persistent: tiles plus staged tables beyond
-// 144 KiB) does not exist.  (Not offered, measured slower or no faster on
-// every grid: a form that keeps whole t-lines in one wave's registers,
-// exchanging through wave-private LDS, 15-40 % slower; 768 and 1024
-// threads per tile, i.e. three and four waves per SIMD.)
+// 144 KiB) does not exist, nor persistent variants of lines longer than 256
+// samples.  (Not offered, measured slower or no faster on every grid: a form
+// that keeps whole t-lines in one wave's registers, exchanging through
+// wave-private LDS, 15-40 % slower; 768 and 1024 threads per tile, i.e.
+// three and four waves per SIMD; persistent workgroups on 512- to
+// 2048-sample lines, 23-90 % slower, their prefetch registers spilling
+// beside 32- and 64-point register DFTs, also with only the twiddles
+// staged so that 4-pixel tiles fit.)
 constexpr int kTimeClassic = 9;
 constexpr int kTimeVariants = kTimeClassic + 18;
 template <typename T, int NT, int V> struct TimeVariant {
@@ -68,7 +72,7 @@ template <typename T, int NT, int V> struct TimeVariant {
     static constexpr int table_bytes = 8 * NT * (int)sizeof(cplx<T>);  // twiddles + 3 tables, hi/lo
     static constexpr bool valid =
         V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 114688 &&
-                   (!persistent || data_bytes + table_bytes <= 147456));
+                   (!persistent || (NT <= 256 && data_bytes + table_bytes <= 147456)));
 };
 // Untuned choice: the default tile with 512 threads where it exists.
 constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 64) ? 1 : 0; }
