@@ -27,6 +27,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "cell-steps/s & %HBM roofline; p50/p99 latency of batched OPO sweeps"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12   # FP32 VALU lane operations, no packed math
 # fields touched per launch of each stage kind
 STAGE_FIELDS = {"nonlinear": 3, "time_op": 3, "xy_op": 3, "free_space": 2, "gdd": 2, "power": 3}
 STAGE_RW = {"power": 1}  # the power reduction only reads
@@ -129,6 +130,7 @@ def main():
                                "(multiplier 1), GDD, free space; step = one round trip",
                    "grid": [grid.nx, grid.ny, grid.nt], "slices": S},
         "roofline": roofline,
+        "valu": valu_utilisation(grid, dom, d["avg_us"] * 1e-6),
         "stages": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in stages.items()},
         "gpu_kernel_ms_per_trip": gpu_kernel_ms,
@@ -276,18 +278,39 @@ def idler_dumped_leg(co, configs, trips):
             "signal_power_last_trip_W": float(pw[-1, 1]), "idler_power_last_trip_W": float(pw[-1, 2])}
 
 
-def measured_traffic(grid, stage):
-    """HBM bytes per launch of a stage's kernel from the committed counter
-    summary (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
-    128 MiB device copy), or None when no summary covers this grid."""
+def _counter_summary(grid, key, stage):
+    """Figure `key` of a stage's kernel from the committed counter summary
+    of this grid, or None when no summary covers it."""
     import re
     path = os.path.join(ROOT, "profiles", f"counters_cavity_{grid.nx}x{grid.ny}x{grid.nt}.md")
     try:
         text = open(path).read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch:.*\b" + re.escape(stage) + r"=(\d+)", text)
+    m = re.search(key + r":.*\b" + re.escape(stage) + r"=(\d+)", text)
     return int(m.group(1)) if m else None
+
+
+def measured_traffic(grid, stage):
+    """HBM bytes per launch of a stage's kernel from the committed counter
+    summary (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
+    128 MiB device copy), or None when no summary covers this grid."""
+    return _counter_summary(grid, "traffic_bytes_per_launch", stage)
+
+
+def valu_utilisation(grid, stage, avg_s):
+    """The kernels are bound by FP32 vector arithmetic, not by HBM: VALU
+    lane operations per cell (rocprofv3 SQ_INSTS_VALU x 64 / cells, from the
+    committed counter summary) times the cells of a launch, over its mean
+    duration, against 256 CUs x 64 lanes x 2.4 GHz (FP32 without packed
+    math, one lane operation per lane per clock; an FMA counts once)."""
+    ops = _counter_summary(grid, "valu_lane_ops_per_cell", stage)
+    if ops is None or avg_s <= 0:
+        return None
+    cells = grid.nx * grid.ny * grid.nt
+    achieved = ops * cells / avg_s
+    return {"kernel": stage, "lane_ops_per_cell": ops, "achieved": round(achieved / 1e12, 2),
+            "peak": VALU_PEAK_T, "unit": "T lane-ops/s", "frac": round(achieved / 1e12 / VALU_PEAK_T, 4)}
 
 
 def plain_leg(trips=4):
