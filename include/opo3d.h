@@ -209,9 +209,10 @@ int opo_profile_trips(opo_solve *s, int n, double ms[OPO_N_STAGES],
  * shapes are compile-time constants, DtypesConsts.cuh:45-48).  For the
  * solve's grid and precision, times every compiled variant of the fused
  * t pass (tile width x block size, one tile per workgroup or persistent
- * workgroups) and of the xy pass (block size, persistent or not) on a
- * stretch of up to 16 slices of the real schedule, `reps` stretches each,
- * by coordinate descent (t, xy, t), and keeps the fastest for all later
+ * workgroups) and of the xy pass (block size, persistent workgroups or
+ * groups of small planes) on stretches of up to 16 slices of the real
+ * schedule, the fastest of `reps` stretches counting, by coordinate
+ * descent (t, xy, t), and keeps the fastest for all later
  * launches of that size in this process.  The solve's fields are saved and
  * restored.  Results are bitwise independent of the variant (tiles only
  * group lines; every multiply-add is explicit).  report (nullable)
