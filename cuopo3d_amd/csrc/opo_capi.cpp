@@ -651,6 +651,44 @@ template <typename T> struct SolveT final : SolveImpl {
         return rc;
     }
 
+    // Fastest of reps timed runs of the stretch (robust against other work
+    // sharing the device during one of them).  The stretch is captured as a
+    // HIP graph and replayed, as a round trip is, unless graphs are off.
+    template <typename F> int time_stretch(F &stretch, int reps, hipEvent_t ev[2], float &ms) {
+        hipGraphExec_t ge = nullptr;
+        if (use_graphs) {
+            hipGraph_t gr = nullptr;
+            CK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+            const int rc = stretch();
+            const hipError_t e = hipStreamEndCapture(stream, &gr);
+            if (rc) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(OPO_ERR_HIP, "autotune capture: %s", hipGetErrorString(e));
+            const hipError_t ei = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ei != hipSuccess) return fail(OPO_ERR_HIP, "autotune instantiate: %s", hipGetErrorString(ei));
+        }
+        auto once = [&]() -> int {
+            if (ge) return hipGraphLaunch(ge, stream) == hipSuccess ? OPO_OK : fail(OPO_ERR_HIP, "autotune launch");
+            return stretch();
+        };
+        int rc = once();   // warm-up
+        for (int r = 0; r < reps && !rc; ++r) {
+            hipError_t e = hipEventRecord(ev[0], stream);
+            if (e == hipSuccess) rc = once();
+            if (!rc && e == hipSuccess) e = hipEventRecord(ev[1], stream);
+            if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[1]);
+            float t = 0.f;
+            if (!rc && e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+            if (!rc && e != hipSuccess) rc = fail(OPO_ERR_HIP, "autotune timing: %s", hipGetErrorString(e));
+            if (!rc) ms = t < ms ? t : ms;
+        }
+        if (ge) (void)hipGraphExecDestroy(ge);
+        return rc;
+    }
+
     // The measurement of autotune(); scratch is owned by the caller.
     int autotune_run(int reps, std::string &report, C *save[3], hipEvent_t ev[2], bool &saved) {
         const bool fp64 = sizeof(T) == 8;
@@ -689,20 +727,9 @@ template <typename T> struct SolveT final : SolveImpl {
                 if (!tune_variant_valid(fp64, axis, n, v)) continue;
                 if (axis == 0 && (g.nx == 1 || g.nx % tune_tile_px(fp64, n, v) != 0)) continue;
                 tune_set(fp64, axis, n, v);
-                int rc = stretch();
-                if (rc) return rc;
-                // fastest of reps timed stretches: robust against other work
-                // that shares the device during one of them
                 float ms = 1e30f;
-                for (int r = 0; r < reps; ++r) {
-                    CK(hipEventRecord(ev[0], stream));
-                    if ((rc = stretch())) return rc;
-                    CK(hipEventRecord(ev[1], stream));
-                    CK(hipEventSynchronize(ev[1]));
-                    float t = 0.f;
-                    CK(hipEventElapsedTime(&t, ev[0], ev[1]));
-                    ms = t < ms ? t : ms;
-                }
+                const int rc = time_stretch(stretch, reps, ev, ms);
+                if (rc) return rc;
                 const double us = 1e3 * ms / (double)L;
                 const int pc = tune_per_cu(fp64, axis, n, v);
                 char pers[32] = "";
