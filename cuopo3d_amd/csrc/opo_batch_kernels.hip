@@ -36,15 +36,10 @@ __device__ __forceinline__ Rk4Args<T> batch_rk(const SolveDesc<T> &d, int mode) 
     return r;
 }
 
-template <typename T, int NT, int TX, int BS>
-__global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ sd,
-                                              const int *__restrict__ status,
-                                              const WorkItem *__restrict__ work, int mode) {
-    __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
-    __shared__ cplx<T> twl[time_stage_twiddles<T, NT, TX>() ? 2 * NT : 1];
-    const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
-    if (status[w.solve]) return;
-    const SolveDesc<T> &d = sd[w.solve];
+// Arguments of the fused pass of one solve in a batch (mode: BATCH_FIRST,
+// BATCH_MID or BATCH_LAST).
+template <typename T, int NT>
+__device__ __forceinline__ TLine3Args<T> batch_time_args(const SolveDesc<T> &d, int mode) {
     TLine3Args<T> a;
     a.psrc = mode == BATCH_FIRST ? d.api : d.f[0];
     a.f[0] = d.f[0];
@@ -60,7 +55,19 @@ __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ s
         a.first_field = 0;
     }
     a.rk = batch_rk(d, mode);
-    nl_time_tile<T, NT, TX, BS, false>(a, d.nx, d.ny, w.item, d.twt, lds, twl);
+    return a;
+}
+
+template <typename T, int NT, int TX, int BS>
+__global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ sd,
+                                              const int *__restrict__ status,
+                                              const WorkItem *__restrict__ work, int mode) {
+    __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
+    __shared__ cplx<T> twl[time_stage_twiddles<T, NT, TX>() ? 2 * NT : 1];
+    const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
+    if (status[w.solve]) return;
+    const SolveDesc<T> &d = sd[w.solve];
+    nl_time_tile<T, NT, TX, BS, false>(batch_time_args<T, NT>(d, mode), d.nx, d.ny, w.item, d.twt, lds, twl);
 }
 
 // Work units of the fused pass over a work list: tile w.item of solve w.solve.
@@ -75,25 +82,7 @@ template <typename T, int NT, int TX> struct TimeBatch {
     __device__ int nx(int u) const { return sd[work[u].solve].nx; }
     __device__ int ny(int u) const { return sd[work[u].solve].ny; }
     __device__ int tile(int u) const { return work[u].item; }
-    __device__ TLine3Args<T> args(int u) const {
-        const SolveDesc<T> &d = sd[work[u].solve];
-        TLine3Args<T> a;
-        a.psrc = mode == BATCH_FIRST ? d.api : d.f[0];
-        a.f[0] = d.f[0];
-        a.f[1] = d.f[1];
-        a.f[2] = d.f[2];
-        if (mode == BATCH_LAST) {
-            a.tab0 = d.tabG;
-            a.tab_stride = 0;
-            a.first_field = d.gdd_on ? 1 : 3;
-        } else {
-            a.tab0 = d.tabT;
-            a.tab_stride = 2 * NT;
-            a.first_field = 0;
-        }
-        a.rk = batch_rk(d, mode);
-        return a;
-    }
+    __device__ TLine3Args<T> args(int u) const { return batch_time_args<T, NT>(sd[work[u].solve], mode); }
     __device__ void stage(int u, cplx<T> *stab, int tid, int nthr) const {
         stage_time_tables<T, NT>(sd[work[u].solve].twt, args(u), stab, tid, nthr);
     }

@@ -56,7 +56,10 @@ template <typename T, int NT, int TX> struct TimeTile {
 // three and four waves per SIMD; persistent workgroups on 512- to
 // 2048-sample lines, 23-90 % slower, their prefetch registers spilling
 // beside 32- and 64-point register DFTs, also with only the twiddles
-// staged so that 4-pixel tiles fit.)
+// staged so that 4-pixel tiles fit; for 2048-sample lines, 4- and 8-pixel
+// tiles with one field in LDS at a time and the other two held in
+// registers, 28-240 % slower, the held fields spilling beside the
+// three-level line operator.)
 constexpr int kTimeClassic = 9;
 constexpr int kTimeVariants = kTimeClassic + 18;
 template <typename T, int NT, int V> struct TimeVariant {
