@@ -3,14 +3,15 @@
 // the OPO on the GPU through the C ABI, and writes the signal field in the
 // reference's text format plus the per-trip powers.
 //
-// usage: opo3d_main POWER_mW WAIST_um [NX NT TRIPS OUTDIR [--all]]
+// usage: opo3d_main POWER_mW WAIST_um [NX NT TRIPS OUTDIR [--all] [--npy]]
 //   Crystal and cavity as in cuOPO3D.cu:19-46 (MgO:sPPLT, 29 mm, T = 27 C,
 //   Lambda = 6.97 um, Lcav = 4 Lcr, Rs = Ri = 0.98, 100 % GDD compensation,
 //   focused-cw pump at 532 nm focused at Lcr/2, signal and idler resonant).
 //   The grid is a runtime choice here (the reference fixes 128x128x2048 at
 //   compile time).  --all writes every time slice and every pixel's time
 //   trace as the reference does (cuOPO3D.cu:75-85); by default only the
-//   slice NT/2 and the trace of the centre pixel are written.
+//   slice NT/2 and the trace of the centre pixel are written.  --npy also
+//   writes the three whole fields as NumPy .npy files.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -23,7 +24,7 @@
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        std::fprintf(stderr, "usage: %s POWER_mW WAIST_um [NX NT TRIPS OUTDIR [--all]]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s POWER_mW WAIST_um [NX NT TRIPS OUTDIR [--all] [--npy]]\n", argv[0]);
         return 2;
     }
     const double power = std::atof(argv[1]) * 1e-3;  // cuOPO3D.cu:36 (mW -> W)
@@ -32,7 +33,11 @@ int main(int argc, char **argv) {
     const int nt = argc > 4 ? std::atoi(argv[4]) : 256;
     const int trips = argc > 5 ? std::atoi(argv[5]) : 200;
     const std::string out = argc > 6 ? std::string(argv[6]) + "/" : std::string("./");
-    const bool all = argc > 7 && std::strcmp(argv[7], "--all") == 0;
+    bool all = false, npy = false;   // --all: every slice and pixel as .dat; --npy: whole fields as .npy
+    for (int k = 7; k < argc; ++k) {
+        all = all || std::strcmp(argv[k], "--all") == 0;
+        npy = npy || std::strcmp(argv[k], "--npy") == 0;
+    }
     try {
         if (opo_device_count() < 1) throw std::runtime_error("no HIP device");
         opo_grid g{nx, nx, nt};
@@ -94,6 +99,11 @@ int main(int argc, char **argv) {
         } else {
             opo3d::saveMatrixComplex_TimeSlice(sig, g, nt / 2, out + "OutputSignalPower" + tag + "_slice" + std::to_string(nt / 2));
             opo3d::SaveElectricField3D_time(sig, g, nx / 2, nx / 2, out + "As_temporal_x_" + std::to_string(nx / 2) + "_y_" + std::to_string(nx / 2));
+        }
+        if (npy) {
+            opo3d::saveNpy(ap, g, out + "pump" + tag + ".npy");
+            opo3d::saveNpy(sig, g, out + "signal" + tag + ".npy");
+            opo3d::saveNpy(idl, g, out + "idler" + tag + ".npy");
         }
         std::printf("trips %zu, final signal power %.6e W\n", pw.size() / 3, pw.empty() ? 0.0 : pw[pw.size() - 2]);
         return rc == OPO_OK ? 0 : 1;

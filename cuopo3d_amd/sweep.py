@@ -10,13 +10,14 @@ same folder layout:
     OUT/waist_<W>/sPPLT_N_<N>/power_per_trip.dat      trip, Pp, Ps, Pi
     OUT/waist_<W>/sPPLT_N_<N>/OutputSignal..._r.dat   x-y slice NT/2 (and _i)
     OUT/waist_<W>/sPPLT_N_<N>/As_temporal_..._r.dat   time trace, centre pixel
+    OUT/waist_<W>/sPPLT_N_<N>/{pump,signal,idler}.npy  whole fields (--npy)
 
 Crystal and cavity follow src/cuOPO3D.cu:19-46 (MgO:sPPLT, 29 mm, 27 C,
 Lambda = 6.97 um, Lcav = 4 Lcr, Rs = Ri = 0.98, GDD compensation, focused
 CW pump focused at Lcr/2, signal and idler resonant, Ai = As noise).
 
 usage: python -m cuopo3d_amd.sweep --powers 500:600:50 --waists 31 \
-           [--grid 64 256] [--trips 200] [--out DIR] [--budget-gb 128]
+           [--grid 64 256] [--trips 200] [--out DIR] [--budget-gb 128] [--npy]
 Ranges are START:STOP:STEP inclusive, like the awk loops of cuOPO3D.sh.
 """
 import argparse
@@ -86,6 +87,7 @@ def main(argv=None):
     ap.add_argument("--trips", type=int, default=200)
     ap.add_argument("--out", default=".")
     ap.add_argument("--budget-gb", type=float, default=128.0)
+    ap.add_argument("--npy", action="store_true", help="also save the three whole fields as .npy")
     args = ap.parse_args(argv)
 
     grid = Grid(args.grid[0], args.grid[0], args.grid[1])
@@ -119,6 +121,9 @@ def main(argv=None):
                       as_[grid.nt // 2], rows=True)
             c = grid.nx // 2
             write_dat(os.path.join(folder, f"As_temporal_x_{c}_y_{c}"), as_[:, c, c], rows=False)
+            if args.npy:   # shape (NT, NY, NX), complex64, the solver's [t][y][x] layout
+                for name, fld in (("pump", ap_), ("signal", as_), ("idler", ai_)):
+                    np.save(os.path.join(folder, f"{name}.npy"), fld)
             done[t] = power[-1, 1]
             print(f"  P = {_fmt(pw)} mW, w0 = {_fmt(w)} um: signal power {power[-1, 1]:.6e} W -> {folder}",
                   flush=True)

@@ -112,4 +112,30 @@ void SaveElectricField3D_time(const std::vector<C> &V, const opo_grid &g, int x,
     }
 }
 
+// Binary alternative to the text writers: the whole field as a NumPy .npy
+// file (format version 1.0), shape (NT, NY, NX), little-endian complex64
+// ("<c8") or complex128 ("<c16") in the [t][y][x] layout of the solver.
+template <typename C>
+void saveNpy(const std::vector<C> &V, const opo_grid &g, const std::string &path) {
+    static_assert(sizeof(C) == 8 || sizeof(C) == 16, "complex<float> or complex<double>");
+    std::string header = std::string("{'descr': '") + (sizeof(C) == 8 ? "<c8" : "<c16") +
+                         "', 'fortran_order': False, 'shape': (" + std::to_string(g.nt) + ", " +
+                         std::to_string(g.ny) + ", " + std::to_string(g.nx) + "), }";
+    // magic (6) + version (2) + header length (2) + header, padded with
+    // spaces to a multiple of 64 bytes and ended by a newline
+    const size_t total = 10 + header.size() + 1;
+    header.append((64 - total % 64) % 64, ' ');
+    header.push_back('\n');
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("saveNpy: cannot open " + path);
+    const unsigned char pre[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    f.write(reinterpret_cast<const char *>(pre), 8);
+    const unsigned short hl = (unsigned short)header.size();
+    const unsigned char len[2] = {(unsigned char)(hl & 0xff), (unsigned char)(hl >> 8)};
+    f.write(reinterpret_cast<const char *>(len), 2);
+    f.write(header.data(), (std::streamsize)header.size());
+    f.write(reinterpret_cast<const char *>(V.data()), (std::streamsize)(V.size() * sizeof(C)));
+    if (!f) throw std::runtime_error("saveNpy: write failed for " + path);
+}
+
 }  // namespace opo3d

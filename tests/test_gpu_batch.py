@@ -167,7 +167,7 @@ def test_sweep_cli_matches_single_solves(tmp_path, require_gpu):
     from cuopo3d_amd import sweep
     from cuopo3d_amd.capi import Grid
     rc = sweep.main(["--powers", "500:600:100", "--waists", "31,40", "--grid", "32", "64",
-                     "--trips", "3", "--out", str(tmp_path)])
+                     "--trips", "3", "--out", str(tmp_path), "--npy"])
     assert rc == 0
     grid = Grid(32, 32, 64)
     ph = sweep.reference_physical(3)
@@ -184,6 +184,11 @@ def test_sweep_cli_matches_single_solves(tmp_path, require_gpu):
             ref = s.runOPO(3)
             # the file holds 17 significant digits of the batch's doubles
             assert np.allclose(rows[:, 1:], ref, rtol=1e-15, atol=0), (w, pw)
+            # --npy: the whole fields, bitwise those of the single solve
+            for name, fld in zip(("pump", "signal", "idler"), s.get_fields()):
+                saved = np.load(folder / f"{name}.npy")
+                assert saved.shape == (64, 32, 32) and np.array_equal(saved, fld), (w, pw, name)
+            s.close()
 
 
 def test_batch_with_eom_equals_single_solves(require_gpu):
@@ -219,14 +224,19 @@ def test_cpp_driver_runs(tmp_path, require_gpu):
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "apps", "opo3d_main")
     if not os.path.exists(exe):
         pytest.skip("apps/opo3d_main is not built (make -C apps)")
-    r = subprocess.run([exe, "500", "31", "32", "64", "3", str(tmp_path)], capture_output=True, text=True,
-                       timeout=300)
+    r = subprocess.run([exe, "500", "31", "32", "64", "3", str(tmp_path), "--npy"], capture_output=True,
+                       text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert "trips 3" in r.stdout
     rows = np.loadtxt(tmp_path / "power_per_trip.dat")
     assert rows.shape == (3, 4) and np.all(np.isfinite(rows))
     assert (tmp_path / "OutputSignalPower_PumpPower500_W_slice32_r.dat").exists()
     assert (tmp_path / "As_temporal_x_16_y_16_i.dat").exists()
+    # the .npy signal holds the same values as the text slice (20 digits)
+    sig = np.load(tmp_path / "signal_PumpPower500_W.npy")
+    assert sig.shape == (64, 32, 32) and sig.dtype == np.complex64
+    re = np.loadtxt(tmp_path / "OutputSignalPower_PumpPower500_W_slice32_r.dat")
+    assert np.array_equal(re.astype(np.float32), sig[32].real)
 
 
 def test_fp64_batch_equals_single_solves(require_gpu):
