@@ -222,11 +222,9 @@ __global__ void __launch_bounds__(64) kb_power_final(const SolveDesc<T> *__restr
     if (!isfinite(pw[1])) status[s] = k + 1;
 }
 
-int batch_tile_px(int nt, bool fp64) { return tune_tile_px(fp64, nt, tune_get(fp64, 0, nt)); }
-
 bool batch_supported(int nx, int ny, int nt, bool fp64) {
     if (nx != ny) return false;
-    const int tx = batch_tile_px(nt, fp64);
+    const int tx = tune_tile_px(fp64, nt, tune_get(fp64, 0, nt));
     if (tx == 0 || nx % tx != 0) return false;
     switch (nx) {
         case 16: case 32: case 64: case 96: return true;
@@ -237,8 +235,7 @@ bool batch_supported(int nx, int ny, int nt, bool fp64) {
 
 template <typename T, int NT>
 static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const int *status,
-                              const WorkItem *work, int mode) {
-    const int v = tune_get(sizeof(T) == 8, 0, NT);
+                              const WorkItem *work, int mode, int v) {
     hipError_t e = hipErrorInvalidValue;
     static_for<0, kTimeVariants>([&](auto vv) {
         constexpr int V = decltype(vv)::value;
@@ -260,17 +257,17 @@ static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
 
 template <typename T>
 hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc<T> *sd,
-                             const int *status, const WorkItem *work, int mode) {
+                             const int *status, const WorkItem *work, int mode, int variant) {
     if (n_work <= 0) return hipSuccess;
     switch (nt) {
-        case 16: return bt_dispatch<T, 16>(st, n_work, sd, status, work, mode);
-        case 32: return bt_dispatch<T, 32>(st, n_work, sd, status, work, mode);
-        case 64: return bt_dispatch<T, 64>(st, n_work, sd, status, work, mode);
-        case 128: return bt_dispatch<T, 128>(st, n_work, sd, status, work, mode);
-        case 256: return bt_dispatch<T, 256>(st, n_work, sd, status, work, mode);
-        case 512: return bt_dispatch<T, 512>(st, n_work, sd, status, work, mode);
-        case 1024: return bt_dispatch<T, 1024>(st, n_work, sd, status, work, mode);
-        case 2048: return bt_dispatch<T, 2048>(st, n_work, sd, status, work, mode);
+        case 16: return bt_dispatch<T, 16>(st, n_work, sd, status, work, mode, variant);
+        case 32: return bt_dispatch<T, 32>(st, n_work, sd, status, work, mode, variant);
+        case 64: return bt_dispatch<T, 64>(st, n_work, sd, status, work, mode, variant);
+        case 128: return bt_dispatch<T, 128>(st, n_work, sd, status, work, mode, variant);
+        case 256: return bt_dispatch<T, 256>(st, n_work, sd, status, work, mode, variant);
+        case 512: return bt_dispatch<T, 512>(st, n_work, sd, status, work, mode, variant);
+        case 1024: return bt_dispatch<T, 1024>(st, n_work, sd, status, work, mode, variant);
+        case 2048: return bt_dispatch<T, 2048>(st, n_work, sd, status, work, mode, variant);
         default: return hipErrorInvalidValue;
     }
 }
@@ -287,7 +284,7 @@ static hipError_t bx_launch_bs(int bs, hipStream_t st, int n_work, const SolveDe
 
 template <typename T, int NX>
 static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
-                              const int *status, const WorkItem *work, int fs) {
+                              const int *status, const WorkItem *work, int fs, int v) {
     if constexpr (sizeof(T) == 8) {
         constexpr int v = xy_default_variant(NX);
         constexpr int BS = xy_variant_bs(NX, v);
@@ -295,7 +292,7 @@ static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
                            status, work, fs);
         return hipGetLastError();
     } else {
-        const int v = tune_get(false, 1, NX);
+        if (!tune_variant_valid(false, 1, NX, v)) return hipErrorInvalidValue;
         const int bs = xy_variant_bs(NX, v);
         if constexpr (NX >= 64) {
             if (xy_variant_persistent(NX, v)) {
@@ -316,15 +313,15 @@ static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
 template <typename T>
 hipError_t launch_batch_xy(hipStream_t st, int nx, int n_work, const SolveDesc<T> *sd,
                            const TripState *ts, const int *status, const WorkItem *work,
-                           int free_space) {
+                           int free_space, int variant) {
     if (n_work <= 0) return hipSuccess;
     switch (nx) {
-        case 16: return bx_dispatch<T, 16>(st, n_work, sd, ts, status, work, free_space);
-        case 32: return bx_dispatch<T, 32>(st, n_work, sd, ts, status, work, free_space);
-        case 64: return bx_dispatch<T, 64>(st, n_work, sd, ts, status, work, free_space);
-        case 96: return bx_dispatch<T, 96>(st, n_work, sd, ts, status, work, free_space);
+        case 16: return bx_dispatch<T, 16>(st, n_work, sd, ts, status, work, free_space, variant);
+        case 32: return bx_dispatch<T, 32>(st, n_work, sd, ts, status, work, free_space, variant);
+        case 64: return bx_dispatch<T, 64>(st, n_work, sd, ts, status, work, free_space, variant);
+        case 96: return bx_dispatch<T, 96>(st, n_work, sd, ts, status, work, free_space, variant);
         case 128:
-            if constexpr (sizeof(T) == 4) return bx_dispatch<T, 128>(st, n_work, sd, ts, status, work, free_space);
+            if constexpr (sizeof(T) == 4) return bx_dispatch<T, 128>(st, n_work, sd, ts, status, work, free_space, variant);
             else return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
@@ -346,9 +343,10 @@ hipError_t launch_batch_power(hipStream_t st, int n_work, int n_solves, const So
 
 #define OPO_BATCH_INST(T)                                                                            \
     template hipError_t launch_batch_time<T>(hipStream_t, int, int, const SolveDesc<T> *, const int *, \
-                                             const WorkItem *, int);                                 \
+                                             const WorkItem *, int, int);                            \
     template hipError_t launch_batch_xy<T>(hipStream_t, int, int, const SolveDesc<T> *,              \
-                                           const TripState *, const int *, const WorkItem *, int);   \
+                                           const TripState *, const int *, const WorkItem *, int,    \
+                                           int);                                                     \
     template hipError_t launch_batch_power<T>(hipStream_t, int, int, const SolveDesc<T> *,           \
                                               const TripState *, int *, const WorkItem *, double *,  \
                                               int);

@@ -892,9 +892,10 @@ struct opo_batch {
     double *d_partial = nullptr;
     struct Range {
         int cls, off, n;
+        int v;   // launch variant the class's work list was built for
     };
     std::vector<Range> trange, xyrange, fsrange;
-    Range prange{0, 0, 0};
+    Range prange{0, 0, 0, 0};
     int n_slices = -1, max_chunk = 1;
     hipGraphExec_t graph = nullptr;
     bool dirty = true;
@@ -966,9 +967,12 @@ struct opo_batch {
             by_nt[resident[i]->g.nt].push_back(i);
             by_nx[resident[i]->g.nx].push_back(i);
         }
+        // Each class's variant is read once here: the work list and the
+        // launches captured below both follow it, whatever the process-wide
+        // choice becomes meanwhile (a change makes the next step rebuild).
         for (auto &kv : by_nt) {
-            Range r{kv.first, (int)work.size(), 0};
-            const int tx = batch_tile_px(kv.first, fp64);
+            Range r{kv.first, (int)work.size(), 0, tune_get(fp64, 0, kv.first)};
+            const int tx = tune_tile_px(fp64, kv.first, r.v);
             for (int i : kv.second) {
                 const opo_grid &g = resident[i]->g;
                 if (tx == 0 || g.nx % tx != 0)
@@ -982,8 +986,9 @@ struct opo_batch {
         }
         for (auto &kv : by_nx) {
             // one item per group of consecutive planes of a field (xy_group)
-            const int pg = xy_group_tuned(fp64, kv.first);
-            Range r{kv.first, (int)work.size(), 0};
+            const int xv = tune_get(fp64, 1, kv.first);
+            const int pg = xy_group(kv.first, xv);
+            Range r{kv.first, (int)work.size(), 0, xv};
             for (int i : kv.second) {
                 if (resident[i]->g.nt % pg != 0)
                     return fail(OPO_ERR_UNSUPPORTED, "NT=%d is not a multiple of the %d-plane xy group (NX=%d)",
@@ -992,14 +997,14 @@ struct opo_batch {
             }
             r.n = (int)work.size() - r.off;
             xyrange.push_back(r);
-            Range q{kv.first, (int)work.size(), 0};
+            Range q{kv.first, (int)work.size(), 0, xv};
             for (int i : kv.second)
                 if (resident[i]->p.fs_on)
                     for (int it = 0; it < 2 * resident[i]->g.nt / pg; ++it) work.push_back(WorkItem{i, it});
             q.n = (int)work.size() - q.off;
             if (q.n) fsrange.push_back(q);
         }
-        prange = Range{0, (int)work.size(), 0};
+        prange = Range{0, (int)work.size(), 0, 0};
         for (int i = 0; i < n; ++i)
             for (int c = 0; c < chunks_for(resident[i]->g); ++c) work.push_back(WorkItem{i, c});
         prange.n = (int)work.size() - prange.off;
@@ -1027,7 +1032,9 @@ struct opo_batch {
         e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
         if (e != hipSuccess) return fail(OPO_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
-        built_variants = current_variants();
+        built_variants.clear();
+        for (const Range &r : trange) built_variants.push_back(r.v);
+        for (const Range &r : xyrange) built_variants.push_back(r.v);
         dirty = false;
         return OPO_OK;
     }
@@ -1037,14 +1044,14 @@ struct opo_batch {
         for (int k = 0; k < n_slices; ++k) {
             for (const Range &r : trange)
                 CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off,
-                                        k == 0 ? BATCH_FIRST : BATCH_MID));
+                                        k == 0 ? BATCH_FIRST : BATCH_MID, r.v));
             for (const Range &r : xyrange)
-                CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0));
+                CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0, r.v));
         }
         for (const Range &r : trange)
-            CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST));
+            CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST, r.v));
         for (const Range &r : fsrange)
-            CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 1));
+            CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 1, r.v));
         CK(launch_batch_power<T>(stream, prange.n, (int)resident.size(), sd, d_ts, d_status,
                                  d_work + prange.off, d_partial, max_chunk));
         return OPO_OK;

@@ -420,11 +420,6 @@ static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<
     }
 }
 
-int xy_group_tuned(bool fp64, int nx) {
-    if (nx < 16) return 1;
-    return xy_group(nx, fp64 ? xy_default_variant(nx) : tune_get(false, 1, nx));
-}
-
 bool xyplane_supported(int nx, int ny, bool fp64) {
     if (nx != ny) return false;
     switch (nx) {

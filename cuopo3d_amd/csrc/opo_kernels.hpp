@@ -106,8 +106,6 @@ unsigned persistent_grid(int units, int per_cu);  // persistent workgroups per C
 __host__ __device__ constexpr int xy_group(int nx, int v) {
     return (nx > 1 && nx <= 32 && v >= 3 && v < 6) ? 256 / nx : 1;
 }
-// Plane group of the xy variant currently chosen for NX (host side).
-int xy_group_tuned(bool fp64, int nx);
 
 bool tline_supported(int nt, bool fp64);
 bool xyplane_supported(int nx, int ny, bool fp64);
@@ -151,24 +149,26 @@ enum { BATCH_FIRST = 0, BATCH_MID = 1, BATCH_LAST = 2 };
 
 // Fused pass for every work item of one NT class.  mode: BATCH_FIRST (pump
 // from Api, one half step), BATCH_MID (two half steps), BATCH_LAST (one half
-// step, mirror, GDD of s and i when the solve has it on).
+// step, mirror, GDD of s and i when the solve has it on).  variant: the
+// launch variant the work list was built for (its tile width); the
+// process-wide choice may have changed since.
 template <typename T>
 hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc<T> *sd,
-                             const int *status, const WorkItem *work, int mode);
-// xy operator for every work item of one NX class: item = f*nt + t over the
-// three fields (free_space = 0) or over s and i with the free-space tables
-// on solves whose TripState.fs is set (free_space = 1).
+                             const int *status, const WorkItem *work, int mode, int variant);
+// xy operator for every work item of one NX class: item = f*groups + group
+// of xy_group(NX, variant) consecutive planes, over the three fields
+// (free_space = 0) or over s and i with the free-space tables on solves
+// whose TripState.fs is set (free_space = 1).
 template <typename T>
 hipError_t launch_batch_xy(hipStream_t st, int nx, int n_work, const SolveDesc<T> *sd,
                            const TripState *ts, const int *status, const WorkItem *work,
-                           int free_space);
+                           int free_space, int variant);
 // Per-solve power: partial sums (item = chunk) then one workgroup per solve
 // writing series[3*trip + f]; a non-finite signal power sets status.
 template <typename T>
 hipError_t launch_batch_power(hipStream_t st, int n_work, int n_solves, const SolveDesc<T> *sd,
                               const TripState *ts, int *status, const WorkItem *work,
                               double *partial, int max_chunk);
-int batch_tile_px(int nt, bool fp64);
 bool batch_supported(int nx, int ny, int nt, bool fp64);
 
 }  // namespace opo

@@ -156,3 +156,34 @@ def test_trip_graph_follows_tuning_changes(restore_tuning, require_gpu):
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
     assert np.array_equal(np.concatenate(pw), pref)
+
+
+def test_batch_follows_tuning_changes_between_trips(restore_tuning, require_gpu):
+    """A resident batch whose xy variant changes between trips (one plane
+    per workgroup to groups of planes, which changes its work list) rebuilds
+    and gives the fields of a batch that kept one variant, bitwise."""
+    lib = restore_tuning
+    solves = [s for s in batch_mix(4, trips=3) if s[0].nx == 32]
+    assert solves
+    lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 32, 0)
+    ref, _ = batch_run(solves, 3)
+    b = co.Batch(8 << 30)
+    tickets = [b.submit(g, p, *f, 3) for (g, p, ph, f) in solves]
+    b.step()
+    assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 32, 4) == OPO_OK
+    results = {}
+    for _ in range(20):
+        b.step()
+        for t in tickets:
+            if t not in results:
+                r = b.poll(t)
+                if r is not None:
+                    results[t] = r
+        if len(results) == len(tickets):
+            break
+    b.close()
+    for t, (pr, fr) in zip(tickets, ref):
+        pg, fg = results[t]
+        for a, c in zip(fg, fr):
+            assert np.array_equal(a, c)
+        assert np.array_equal(pg, pr)
