@@ -248,6 +248,7 @@ def fp64_leg(co, configs, trips):
     g, p, _, f = configs.cavity_64(n_round_trips=trips + 1, precision=OPO_FP64)
     s = co.Solver(g, p, device=0)
     s.set_fields(*f)
+    s.autotune(reps=3)
     s.runOPO(1)
     s.sync()
     t0 = time.perf_counter()

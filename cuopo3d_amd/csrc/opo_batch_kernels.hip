@@ -285,16 +285,10 @@ static hipError_t bx_launch_bs(int bs, hipStream_t st, int n_work, const SolveDe
 template <typename T, int NX>
 static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
                               const int *status, const WorkItem *work, int fs, int v) {
-    if constexpr (sizeof(T) == 8) {
-        constexpr int v = xy_default_variant(NX);
-        constexpr int BS = xy_variant_bs(NX, v);
-        hipLaunchKernelGGL((kb_xy<T, NX, BS, xy_group(NX, v)>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts,
-                           status, work, fs);
-        return hipGetLastError();
-    } else {
-        if (!tune_variant_valid(false, 1, NX, v)) return hipErrorInvalidValue;
-        const int bs = xy_variant_bs(NX, v);
-        if constexpr (NX >= 64) {
+    if (!tune_variant_valid(sizeof(T) == 8, 1, NX, v)) return hipErrorInvalidValue;
+    const int bs = xy_variant_bs(NX, v);
+    if constexpr (NX >= 64) {
+        if constexpr (sizeof(T) == 4) {   // persistent planes: FP32 only
             if (xy_variant_persistent(NX, v)) {
                 const dim3 pg(persistent_grid(n_work, xy_variant_per_cu(NX, v)));
                 if (bs == 256) hipLaunchKernelGGL((kb_xyp<T, NX, 256>), pg, dim3(256), 0, st, sd, ts, status, work, n_work, fs);
@@ -302,11 +296,11 @@ static hipError_t bx_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
                 else hipLaunchKernelGGL((kb_xyp<T, NX, 1024>), pg, dim3(1024), 0, st, sd, ts, status, work, n_work, fs);
                 return hipGetLastError();
             }
-            return bx_launch_bs<T, NX, 1>(bs, st, n_work, sd, ts, status, work, fs);
-        } else {
-            if (xy_group(NX, v) > 1) return bx_launch_bs<T, NX, xy_group(NX, 3)>(bs, st, n_work, sd, ts, status, work, fs);
-            return bx_launch_bs<T, NX, 1>(bs, st, n_work, sd, ts, status, work, fs);
         }
+        return bx_launch_bs<T, NX, 1>(bs, st, n_work, sd, ts, status, work, fs);
+    } else {
+        if (xy_group(NX, v) > 1) return bx_launch_bs<T, NX, xy_group(NX, 3)>(bs, st, n_work, sd, ts, status, work, fs);
+        return bx_launch_bs<T, NX, 1>(bs, st, n_work, sd, ts, status, work, fs);
     }
 }
 

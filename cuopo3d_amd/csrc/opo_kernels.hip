@@ -245,10 +245,9 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
         return fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
     }
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
-    if (fp64) return v == xy_default_variant(n);
     if (n < 16) return v == 0;
     if (n <= 32) return v < 6;
-    if (xy_variant_persistent(n, v)) return n * n / xy_variant_bs(n, v) <= 16;
+    if (xy_variant_persistent(n, v)) return !fp64 && n * n / xy_variant_bs(n, v) <= 16;
     return true;
 }
 
@@ -398,19 +397,18 @@ static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<
                               const cplx<T> *tw, const int *status) {
     if constexpr (NX < 16) {
         return xy_launch_groups<T, NX, 256, 1>(st, g, ops, nf, tw, status);
-    } else if constexpr (sizeof(T) == 8) {
-        constexpr int v = xy_default_variant(NX);
-        return xy_launch_groups<T, NX, xy_variant_bs(NX, v), xy_group(NX, v)>(st, g, ops, nf, tw, status);
     } else {
-        const int v = tune_get(false, 1, NX);
+        const int v = tune_get(sizeof(T) == 8, 1, NX);
         const int bs = xy_variant_bs(NX, v);
         if constexpr (NX >= 64) {
-            if (xy_variant_persistent(NX, v)) {
-                const dim3 pg(persistent_grid(g.nt * nf, xy_variant_per_cu(NX, v)));
-                if (bs == 256) hipLaunchKernelGGL((k_xyplanep<T, NX, 256>), pg, dim3(256), 0, st, ops, g.nt, nf, tw, status);
-                else if (bs == 512) hipLaunchKernelGGL((k_xyplanep<T, NX, 512>), pg, dim3(512), 0, st, ops, g.nt, nf, tw, status);
-                else hipLaunchKernelGGL((k_xyplanep<T, NX, 1024>), pg, dim3(1024), 0, st, ops, g.nt, nf, tw, status);
-                return hipGetLastError();
+            if constexpr (sizeof(T) == 4) {   // persistent planes: FP32 only
+                if (xy_variant_persistent(NX, v)) {
+                    const dim3 pg(persistent_grid(g.nt * nf, xy_variant_per_cu(NX, v)));
+                    if (bs == 256) hipLaunchKernelGGL((k_xyplanep<T, NX, 256>), pg, dim3(256), 0, st, ops, g.nt, nf, tw, status);
+                    else if (bs == 512) hipLaunchKernelGGL((k_xyplanep<T, NX, 512>), pg, dim3(512), 0, st, ops, g.nt, nf, tw, status);
+                    else hipLaunchKernelGGL((k_xyplanep<T, NX, 1024>), pg, dim3(1024), 0, st, ops, g.nt, nf, tw, status);
+                    return hipGetLastError();
+                }
             }
             return xy_launch_bs<T, NX, 1>(bs, st, g, ops, nf, tw, status);
         } else {

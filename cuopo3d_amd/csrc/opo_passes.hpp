@@ -47,7 +47,7 @@ template <typename T, int NT, int TX> struct TimeTile {
 // tiles).  V >= 9: persistent workgroups (k + 1 per CU), each walking a
 // contiguous run of tiles with the next tile prefetched into registers;
 // V - 9 = 2 (3 m + b) + k with b = 0: 256, 1: 384, 2: 512 threads.
-// Only variant 0 exists for FP64 and for NT < 64; a variant whose three
+// FP64 has the one-tile variants only; NT < 64 only variant 0; a variant whose three
 // tiles exceed 112 KiB of LDS (persistent: tiles plus staged tables beyond
 // 144 KiB) does not exist, nor persistent variants of lines longer than 256
 // samples.  (Not offered, measured slower or no faster on every grid: a form
@@ -74,7 +74,7 @@ template <typename T, int NT, int V> struct TimeVariant {
     static constexpr int data_bytes = 3 * TimeTile<T, NT, (tx > 0 ? tx : 1)>::FT * (int)sizeof(cplx<T>);
     static constexpr int table_bytes = 8 * NT * (int)sizeof(cplx<T>);  // twiddles + 3 tables, hi/lo
     static constexpr bool valid =
-        V == 0 || (sizeof(T) == 4 && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 114688 &&
+        V == 0 || ((sizeof(T) == 4 || !persistent) && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 114688 &&
                    (!persistent || (NT <= 256 && data_bytes + table_bytes <= 147456)));
 };
 // Untuned choice: the default tile with 512 threads where it exists.

@@ -187,3 +187,35 @@ def test_batch_follows_tuning_changes_between_trips(restore_tuning, require_gpu)
         for a, c in zip(fg, fr):
             assert np.array_equal(a, c)
         assert np.array_equal(pg, pr)
+
+
+@pytest.mark.parametrize("n", [(32, 32, 64), (64, 64, 256)], ids=lambda n: "x".join(map(str, n)))
+def test_fp64_variants_give_the_same_fields(n, require_gpu):
+    """The FP64 path's launch variants (one tile or plane per workgroup, or
+    groups of small planes) give bitwise the fields of variant 0."""
+    from cuopo3d_amd.capi import OPO_FP64
+    lib = co.load()
+    nx, ny, nt = n
+    keep = {a: lib.opo_tune_get(OPO_FP64, a, sz) for a, sz in ((OPO_TUNE_TIME_PASS, nt), (OPO_TUNE_XY_PASS, nx))}
+    grid, p, _, fields = configs.cavity_64(precision=OPO_FP64, n=n, n_round_trips=2)
+    try:
+        for axis, size in ((OPO_TUNE_TIME_PASS, nt), (OPO_TUNE_XY_PASS, nx)):
+            lib.opo_tune_set(OPO_FP64, OPO_TUNE_TIME_PASS, nt, 0)
+            lib.opo_tune_set(OPO_FP64, OPO_TUNE_XY_PASS, nx, 0)
+            ref, pref = _run(grid, p, fields)
+            vs = [v for v in range(64) if lib.opo_tune_set(OPO_FP64, axis, size, v) == OPO_OK]
+            assert len(vs) > 1, (axis, size)
+            for v in vs:
+                assert lib.opo_tune_set(OPO_FP64, axis, size, v) == OPO_OK
+                try:
+                    got, pw = _run(grid, p, fields)
+                except RuntimeError as e:   # a tile that does not divide NX is refused, not run
+                    assert "does not divide" in str(e), e
+                    continue
+                for a, b in zip(got, ref):
+                    assert np.array_equal(a, b), (axis, v)
+                assert np.array_equal(pw, pref), (axis, v)
+            print(f"fp64 {n} axis {axis}: variants {vs} bitwise equal", flush=True)
+    finally:
+        for a, v in keep.items():
+            lib.opo_tune_set(OPO_FP64, a, nt if a == OPO_TUNE_TIME_PASS else nx, v)
