@@ -128,6 +128,18 @@ def test_round_trips_nt2048(require_gpu):
     assert rel.max() < 1e-6, rel
 
 
+def test_reference_default_grid_two_slices(require_gpu):
+    """The largest grid, the one the reference compiles by default
+    (128 x 128 x 2048: both the three-level t operator and the 128-point xy
+    lines at full size), two slices of a single pass against the oracle."""
+    grid, p, ph, fields = configs.cavity_64(n=(128, 128, 2048), n_round_trips=1)
+    p.n_slices = 2
+    s, o = _pair(grid, p, fields)
+    s.runSinglePass()
+    o.ssfm()
+    _check(s, o, 1e-5, "2 slices 128x128x2048")
+
+
 @pytest.mark.parametrize("n", [(16, 16, 2048), (32, 32, 1024)], ids=lambda n: "x".join(map(str, n)))
 def test_fp64_long_lines(n, require_gpu):
     """FP64 fused passes on the longest time lines (their line operators
