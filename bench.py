@@ -144,7 +144,9 @@ def main():
         "signal_power_last_trip_W": float(pw[-1, 1]),
     }
 
+    single_fields = {}
     if default_grid:
+        out["single_pass"], single_fields = single_pass_leg(co, configs)
         out["idler_dumped"] = idler_dumped_leg(co, configs, args.steps)
         out["fp64_path"] = fp64_leg(co, configs, min(args.steps, 50))
         if not args.no_reference_grid:
@@ -172,6 +174,29 @@ def main():
                                          f"same 64x64x256 cavity workload ({c1 - c0:.1f} s)"}
         out["parity_vs_oracle"] = {"trips": args.cpu_trips, "field_rel_l2_p_s_i": errs,
                                    "signal_power_rel_max": perr}
+        # the single-pass configs in full on the host, with the GPU's error; the
+        # oracle starts from the FP64 set-up (the FP32 set-up is its rounding)
+        from cuopo3d_amd.capi import OPO_FP64
+        others = {}
+        for (name, prec), gf in single_fields.items():
+            if prec:
+                continue
+            make = configs.single_pass_32 if name.startswith("single") else configs.femto_64
+            g3, p3, _, f3 = make(precision=OPO_FP64)
+            o3 = Oracle(g3, p3)
+            o3.set_fields(*f3)
+            c2 = time.perf_counter()
+            o3.ssfm()
+            c3 = time.perf_counter()
+            rf3 = o3.get_fields()
+            entry = {"value": g3.nx * g3.ny * g3.nt * p3.n_slices / (c3 - c2), "unit": "cell-steps/s",
+                     "seconds": c3 - c2, "cores": get_threads()}
+            for (n2, q), fl in single_fields.items():
+                if n2 == name:
+                    entry["fp64_rel_l2_p_s_i" if q else "fp32_rel_l2_p_s_i"] = [
+                        float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fl, rf3)]
+            others[name] = entry
+        out["cpu_baseline"]["single_pass_configs"] = others
     if not args.no_plain:
         out["plain_rocfft"] = plain_leg()
         if "value" in out["plain_rocfft"]:
@@ -220,6 +245,36 @@ def horizon_leg(co, configs, trips):
     s32.close()
     s64.close()
     return res
+
+
+def single_pass_leg(co, configs):
+    """The single-pass configs of the baseline plan on the GPU: the 32x32x64
+    plain single pass (100 slices) and the femtosecond 64x64x1024 single
+    pass (399 slices, also on the FP64 path).  Returns the JSON summary and
+    the output fields (for the oracle comparison of the CPU leg)."""
+    from cuopo3d_amd.capi import OPO_FP64
+    summary, fields = {}, {}
+    for name, make, reps, precs in (("single_pass_32x32x64", configs.single_pass_32, 20, (0,)),
+                                    ("femto_64x64x1024", configs.femto_64, 3, (0, OPO_FP64))):
+        for prec in precs:
+            g, p, _, f = make(precision=prec)
+            s = co.Solver(g, p, device=0)
+            s.set_fields(*f)
+            s.autotune(reps=3)
+            ts = []
+            for _ in range(reps + 1):   # the first pass warms up
+                s.set_fields(*f)
+                s.sync()
+                t0 = time.perf_counter()
+                s.runSinglePass()       # returns after the pass has finished on the GPU
+                ts.append(time.perf_counter() - t0)
+            dt = float(np.median(ts[1:]))
+            fields[(name, prec)] = s.get_fields()
+            s.close()
+            leg = summary.setdefault(name, {"grid": [g.nx, g.ny, g.nt], "slices": p.n_slices})
+            leg["fp64" if prec else "fp32"] = {"value": g.nx * g.ny * g.nt * p.n_slices / dt,
+                                               "unit": "cell-steps/s", "ms_per_pass": dt * 1e3}
+    return summary, fields
 
 
 def reference_grid_leg(co, configs, tune_reps, trips=3):

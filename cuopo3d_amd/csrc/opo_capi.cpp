@@ -147,7 +147,9 @@ template <typename T> struct SolveT final : SolveImpl {
     int series_cap = 0;
     int *flags = nullptr;     // [0] status, [1] trip counter
     int nblk = 64;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // Captured work: [0] a round trip with free space, [1] one without,
+    // [2] a single pass (opo_single_pass).
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     int graph_variants[2] = {-1, -1};   // (t, xy) launch variants the graphs were captured with
 
     ~SolveT() override {
@@ -538,7 +540,7 @@ template <typename T> struct SolveT final : SolveImpl {
     int ssfm() override {
         int rc = select();
         if (rc) return rc;
-        if ((rc = q_ssfm(false, nullptr))) return rc;
+        if ((rc = launch_graph(2))) return rc;
         CK(hipStreamSynchronize(stream));
         return OPO_OK;
     }
@@ -557,9 +559,10 @@ template <typename T> struct SolveT final : SolveImpl {
         return OPO_OK;
     }
 
-    int launch_trip(bool fs) {
-        int v = fs ? 0 : 1;
-        if (!use_graphs) return q_trip(fs);
+    // Enqueue graph v (see graph[]), capturing it on first use.
+    int launch_graph(int v) {
+        auto enqueue = [&]() { return v == 2 ? q_ssfm(false, nullptr) : q_trip(v == 0); };
+        if (!use_graphs) return enqueue();
         // the launch variants are process-wide: re-capture when they changed
         const bool fp64 = sizeof(T) == 8;
         const int cur[2] = {tune_get(fp64, 0, g.nt), tune_get(fp64, 1, g.nx)};
@@ -572,7 +575,7 @@ template <typename T> struct SolveT final : SolveImpl {
         if (!graph[v]) {
             hipGraph_t gr = nullptr;
             CK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
-            int rc = q_trip(fs);
+            int rc = enqueue();
             hipError_t e = hipStreamEndCapture(stream, &gr);
             if (rc) {
                 if (gr) (void)hipGraphDestroy(gr);
@@ -596,7 +599,7 @@ template <typename T> struct SolveT final : SolveImpl {
         for (int i = 0; i < n; ++i) {
             const int rt = trips_done + i;
             const bool fs = p.fs_on && !(p.skip_fs_on_last_trip && rt == p.n_round_trips - 1);
-            if ((rc = launch_trip(fs))) return rc;
+            if ((rc = launch_graph(fs ? 0 : 1))) return rc;
         }
         pending = n;
         return OPO_OK;
