@@ -197,6 +197,27 @@ def main():
                         float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fl, rf3)]
             others[name] = entry
         out["cpu_baseline"]["single_pass_configs"] = others
+        # the smallest and largest grids of the sweep, 2 round trips each
+        sweep_cpu = {}
+        solves = [configs.sweep_solve(d, n_round_trips=2) for d in configs.sweep_draws(n_solves=64, seed=3)]
+        cells_of = [g.nx * g.ny * g.nt for (g, _, _, _) in solves]
+        for tag, k in (("smallest", int(np.argmin(cells_of))), ("largest", int(np.argmax(cells_of)))):
+            g4, p4, _, f4 = solves[k]
+            o4 = Oracle(g4, p4)
+            o4.set_fields(*f4)
+            c4 = time.perf_counter()
+            pr4 = o4.run_round_trips(2)
+            c5 = time.perf_counter()
+            s4 = co.Solver(g4, p4, device=0)
+            s4.set_fields(*f4)
+            pg4 = s4.runOPO(2)
+            e4 = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(s4.get_fields(), o4.get_fields())]
+            s4.close()
+            sweep_cpu[tag] = {"solve": k, "grid": [g4.nx, g4.ny, g4.nt], "trips": 2,
+                              "value": cells_of[k] * p4.n_slices * 2 / (c5 - c4), "unit": "cell-steps/s",
+                              "seconds": c5 - c4, "field_rel_l2_p_s_i": e4,
+                              "signal_power_rel_max": float(np.max(np.abs(pg4[:, 1] - pr4[:, 1]) / np.abs(pr4[:, 1])))}
+        out["cpu_baseline"]["sweep_grids"] = sweep_cpu
     if not args.no_plain:
         out["plain_rocfft"] = plain_leg()
         if "value" in out["plain_rocfft"]:

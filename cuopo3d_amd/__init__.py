@@ -125,6 +125,12 @@ class Solver:
         check(self.lib.opo_power(self.h, out), "opo_power")
         return np.array(list(out))
 
+    def averagePowerTime(self, field, x, y, prefactor):
+        """prefactor * sum_t |A(x, y, t)|^2 of field 0/1/2 (opo_power_time)."""
+        out = C.c_double()
+        check(self.lib.opo_power_time(self.h, field, x, y, prefactor, C.byref(out)), "opo_power_time")
+        return out.value
+
     def sync(self):
         check(self.lib.opo_sync(self.h), "opo_sync")
 

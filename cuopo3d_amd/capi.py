@@ -141,6 +141,7 @@ def load():
         "opo_run_round_trips": (ip, [vp, ip, dp]),
         "opo_trips_done": (ip, [vp]),
         "opo_power": (ip, [vp, dp]),
+        "opo_power_time": (ip, [vp, C.c_int, C.c_int, C.c_int, C.c_double, dp]),
         "opo_courant_ok": (ip, [C.POINTER(Params)]),
         "opo_sync": (ip, [vp]),
         "opo_profile_trips": (ip, [vp, ip, dp, C.POINTER(ip)]),

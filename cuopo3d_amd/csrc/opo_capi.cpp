@@ -122,6 +122,7 @@ struct SolveImpl {
     virtual int enqueue_trips(int n) = 0;
     virtual int finish_trips(int n, double *power) = 0;
     virtual int power(double out[3]) = 0;
+    virtual int power_time(int f, int x, int y, double pref, double *out) = 0;
     virtual int profile_trips(int n, double *ms, int *launches) = 0;
     // Write this solve's SolveDesc<T> (T = the solve's precision) to out.
     virtual void fill_desc(void *out, double *series, int series_cap, int nchunk) const = 0;
@@ -791,6 +792,16 @@ template <typename T> struct SolveT final : SolveImpl {
         std::memcpy(out, &d, sizeof(d));
     }
 
+    int power_time(int f, int x, int y, double pref, double *out) override {
+        int rc = select();
+        if (rc) return rc;
+        const size_t plane = (size_t)g.nx * g.ny;
+        CK(launch_power_pixel<T>(stream, A[f] + (size_t)y * g.nx + x, plane, g.nt, pref, pw3 + 3));
+        CK(hipMemcpyAsync(out, pw3 + 3, sizeof(double), hipMemcpyDeviceToHost, stream));
+        CK(hipStreamSynchronize(stream));
+        return OPO_OK;
+    }
+
     int power(double out[3]) override {
         int rc = select();
         if (rc) return rc;
@@ -1122,6 +1133,14 @@ int opo_power(opo_solve *s, double out[3]) {
     SOLVE_OR_FAIL(s);
     if (!out) return fail(OPO_ERR_ARG, "null output");
     return s->s->power(out);
+}
+int opo_power_time(opo_solve *s, int field, int x, int y, double prefactor, double *out) {
+    SOLVE_OR_FAIL(s);
+    if (!out) return fail(OPO_ERR_ARG, "null output");
+    if (field < 0 || field > 2) return fail(OPO_ERR_ARG, "field must be 0 (pump), 1 (signal) or 2 (idler)");
+    if (x < 0 || x >= s->s->g.nx || y < 0 || y >= s->s->g.ny)
+        return fail(OPO_ERR_ARG, "pixel (%d, %d) outside the %d x %d grid", x, y, s->s->g.nx, s->s->g.ny);
+    return s->s->power_time(field, x, y, prefactor, out);
 }
 int opo_courant_ok(const opo_params *p) {
     if (!p) return 0;

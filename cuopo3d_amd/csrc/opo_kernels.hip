@@ -529,6 +529,36 @@ hipError_t launch_power_final(hipStream_t st, const double *partial, int nblk, d
     return hipGetLastError();
 }
 
+// Power of one pixel's time series: pref * sum_t |A(x, y, t)|^2, one
+// workgroup, Neumaier-compensated per thread and reduced in a fixed tree
+// order (averagePowerTime, Solver3D.cuh:509-523).
+template <typename T>
+__global__ void __launch_bounds__(256) k_power_pixel(const cplx<T> *f, size_t plane, int nt,
+                                                     double pref, double *out) {
+    __shared__ double red[256];
+    const int tid = threadIdx.x;
+    double s = 0.0, c = 0.0;
+    for (int t = tid; t < nt; t += 256) {
+        const cplx<T> a = f[(size_t)t * plane];
+        const double re = (double)a.x, im = (double)a.y;
+        neumaier_add(s, c, fma(re, re, im * im));
+    }
+    red[tid] = s + c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] += red[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) out[0] = pref * red[0];
+}
+
+template <typename T>
+hipError_t launch_power_pixel(hipStream_t st, const cplx<T> *f, size_t plane, int nt, double pref,
+                              double *out) {
+    hipLaunchKernelGGL((k_power_pixel<T>), dim3(1), dim3(256), 0, st, f, plane, nt, pref, out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------- instantiation
 #define OPO_INST(T)                                                                               \
     template hipError_t launch_rk4<T>(hipStream_t, const cplx<T> *, cplx<T> *, cplx<T> *,          \
@@ -540,7 +570,9 @@ hipError_t launch_power_final(hipStream_t st, const double *partial, int nblk, d
     template hipError_t launch_xyplane<T>(hipStream_t, const Geometry &, const FieldOps<T> &, int, \
                                           const cplx<T> *, const int *);                          \
     template hipError_t launch_power_partial<T>(hipStream_t, const cplx<T> *, const cplx<T> *,     \
-                                                const cplx<T> *, size_t, int, double *, const int *);
+                                                const cplx<T> *, size_t, int, double *, const int *); \
+    template hipError_t launch_power_pixel<T>(hipStream_t, const cplx<T> *, size_t, int, double,   \
+                                              double *);
 OPO_INST(float)
 OPO_INST(double)
 

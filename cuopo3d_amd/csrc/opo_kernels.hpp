@@ -84,6 +84,12 @@ hipError_t launch_power_final(hipStream_t st, const double *partial, int nblk, d
                               double pref1, double pref2, double *series, int *trip, int *status,
                               int check_nan, int series_cap);
 
+// pref * sum_t |f[t*plane]|^2 into out[0] (one workgroup; f points at the
+// pixel's t = 0 sample).
+template <typename T>
+hipError_t launch_power_pixel(hipStream_t st, const cplx<T> *f, size_t plane, int nt, double pref,
+                              double *out);
+
 // Launch-variant tables (the autotuner's choices), per precision and size.
 // axis 0: fused t pass, keyed by NT; axis 1: xy pass, keyed by NX.
 int tune_variant_count(int axis);

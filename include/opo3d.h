@@ -182,6 +182,13 @@ int opo_run_round_trips(opo_solve *s, int n, double *power);
 int opo_trips_done(const opo_solve *s);
 /* averagePowerCutXY(A) for p, s, i (Solver3D.cuh:529-545), compensated. */
 int opo_power(opo_solve *s, double out[3]);
+/* averagePowerTime(A, x, y) (Solver3D.cuh:509-523): *out = prefactor *
+ * sum_t |A(x, y, t)|^2 of field 0 (p), 1 (s) or 2 (i), compensated FP64
+ * sum over the NT samples of the pixel.  The reference's prefactor is
+ * 0.5*EPS0*C*ns*dt/trt (Solver3D.cuh:521); its reduction runs over NX*NY*NT
+ * entries of an NT-entry buffer, past its end, so only the NT samples it
+ * wrote are summed here. */
+int opo_power_time(opo_solve *s, int field, int x, int y, double prefactor, double *out);
 /* checkCourantStability(dz, 0.2*kp*dx*dy) (Solver3D.cuh:495-504), with kp
  * = kdiff[0].  Returns 1 when passed, 0 when failed. */
 int opo_courant_ok(const opo_params *p);

@@ -67,6 +67,7 @@ def test_null_handles_are_errors():
     assert lib.opo_run_round_trips(None, 1, None) == OPO_ERR_ARG
     assert lib.opo_batch_step(None, None) == OPO_ERR_ARG
     assert lib.opo_field_bytes(None) == 0
+    assert lib.opo_power_time(None, 0, 0, 0, 1.0, None) == OPO_ERR_ARG
     lib.opo_destroy(None)
 
 

@@ -251,3 +251,28 @@ def test_plane_wave_cavity(gdd_on, require_gpu):
     # field (~5e-6 rel L2) to average out, so the 3-D 1e-6 bound is relaxed
     # to twice the field error's power equivalent here.
     assert rel.max() < 3e-6, rel
+
+
+@pytest.mark.parametrize("n,prec", [((32, 32, 64), OPO_FP32), ((16, 16, 2048), OPO_FP32),
+                                    ((32, 32, 64), OPO_FP64)],
+                         ids=["32x32x64", "16x16x2048", "32x32x64-fp64"])
+def test_power_time_of_a_pixel(n, prec, require_gpu):
+    """averagePowerTime: prefactor * sum_t |A(x, y, t)|^2 against an FP64
+    sum of the fields read back (squares of FP32 values are exact in FP64,
+    so both sums agree to a few FP64 ulps)."""
+    grid, p, _ = cavity_params(n, precision=prec)
+    s = co.Solver(grid, p, device=0)
+    s.set_fields(*random_fields(grid, 21, precision=prec))
+    s.dispersion()
+    f = s.get_fields()
+    pref = 0.37
+    for x, y in [(0, 0), (n[0] - 1, n[1] - 1), (n[0] // 2, 3), (5, n[1] // 2 + 1)]:
+        for k in range(3):
+            want = pref * float(np.sum(np.abs(f[k][:, y, x].astype(np.complex128)) ** 2))
+            got = s.averagePowerTime(k, x, y, pref)
+            assert abs(got - want) <= 1e-13 * want, (k, x, y, got, want)
+    with pytest.raises(RuntimeError):
+        s.averagePowerTime(0, n[0], 0, pref)
+    with pytest.raises(RuntimeError):
+        s.averagePowerTime(3, 0, 0, pref)
+    s.close()
