@@ -131,6 +131,10 @@ class Solver:
         check(self.lib.opo_power_time(self.h, field, x, y, prefactor, C.byref(out)), "opo_power_time")
         return out.value
 
+    def checkCourantStability(self):
+        """dz < 0.2 kp dx dy (opo_courant_ok); True when it holds."""
+        return bool(self.lib.opo_courant_ok(C.byref(self.params)))
+
     def sync(self):
         check(self.lib.opo_sync(self.h), "opo_sync")
 

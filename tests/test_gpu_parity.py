@@ -275,4 +275,6 @@ def test_power_time_of_a_pixel(n, prec, require_gpu):
         s.averagePowerTime(0, n[0], 0, pref)
     with pytest.raises(RuntimeError):
         s.averagePowerTime(3, 0, 0, pref)
+    # the Courant gate of the solve's own parameters
+    assert s.checkCourantStability() == (p.dz < 0.2 * p.kdiff[0] * p.dx * p.dy)
     s.close()
