@@ -281,7 +281,8 @@ def single_pass_leg(co, configs):
             g, p, _, f = make(precision=prec)
             s = co.Solver(g, p, device=0)
             s.set_fields(*f)
-            s.autotune(reps=3)
+            for line in s.autotune(reps=3).strip().splitlines():
+                print(f"{name} autotune:", line.strip(), file=sys.stderr)
             ts = []
             for _ in range(reps + 1):   # the first pass warms up
                 s.set_fields(*f)
