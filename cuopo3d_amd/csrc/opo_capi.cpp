@@ -541,7 +541,7 @@ template <typename T> struct SolveT final : SolveImpl {
     int ssfm() override {
         int rc = select();
         if (rc) return rc;
-        if ((rc = launch_graph(2))) return rc;
+        if (p.n_slices > 0 && (rc = launch_graph(2))) return rc;   // no slices: nothing to capture
         CK(hipStreamSynchronize(stream));
         return OPO_OK;
     }
