@@ -278,3 +278,16 @@ def test_power_time_of_a_pixel(n, prec, require_gpu):
     # the Courant gate of the solve's own parameters
     assert s.checkCourantStability() == (p.dz < 0.2 * p.kdiff[0] * p.dx * p.dy)
     s.close()
+
+
+def test_single_pass_without_slices_leaves_fields(require_gpu):
+    """n_slices = 0: the single pass has nothing to do and changes nothing."""
+    grid, p, _ = cavity_params((32, 32, 64))
+    p.n_slices = 0
+    f = random_fields(grid, 23)
+    s = co.Solver(grid, p, device=0)
+    s.set_fields(*f)
+    s.runSinglePass()
+    for a, b in zip(s.get_fields(), f):
+        assert np.array_equal(a, b)
+    s.close()
