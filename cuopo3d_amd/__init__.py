@@ -228,5 +228,5 @@ class Batch:
               "opo_batch_stats")
         return r.value, q.value, u.value
 
-    def footprint(self, grid):
-        return self.lib.opo_batch_footprint(C.byref(grid), self.precision)
+    def footprint(self, grid, n_trips=1):
+        return self.lib.opo_batch_footprint(C.byref(grid), self.precision, int(n_trips))

@@ -154,7 +154,7 @@ def load():
                                   C.POINTER(C.c_uint64)]),
         "opo_batch_step": (ip, [vp, C.POINTER(ip)]),
         "opo_batch_poll": (ip, [vp, C.c_uint64, C.POINTER(ip), dp, vp, vp, vp]),
-        "opo_batch_footprint": (sz, [C.POINTER(Grid), ip]),
+        "opo_batch_footprint": (sz, [C.POINTER(Grid), ip, ip]),
         "opo_batch_stats": (ip, [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(sz)]),
         # host set-up (include/opo3d_setup.h)
         "opo_slice_count": (ip, [C.c_double, ip]),

@@ -223,9 +223,7 @@ __global__ void __launch_bounds__(64) kb_power_final(const SolveDesc<T> *__restr
 }
 
 bool batch_supported(int nx, int ny, int nt, bool fp64) {
-    if (nx != ny) return false;
-    const int tx = tune_tile_px(fp64, nt, tune_get(fp64, 0, nt));
-    if (tx == 0 || nx % tx != 0) return false;
+    if (nx != ny || !tline_supported(nt, fp64) || nt < 16) return false;
     switch (nx) {
         case 16: case 32: case 64: case 96: return true;
         case 128: return !fp64;

@@ -180,9 +180,7 @@ template <typename T> struct SolveT final : SolveImpl {
             if (hipMalloc(&a, fb) != hipSuccess) return fail(OPO_ERR_NOMEM, "hipMalloc(%zu) failed", fb);
         CK(hipMemsetAsync(Api, 0, fb, stream));
         for (auto *a : A) CK(hipMemsetAsync(a, 0, fb, stream));
-        // power partials: enough blocks to spread the sum, fixed per grid.
-        size_t want = (N + 256 * 16 - 1) / (256 * 16);
-        nblk = (int)(want < 1 ? 1 : (want > 1024 ? 1024 : want));
+        nblk = power_blocks(N);
         CK(hipMalloc(&dbuf, sizeof(double) * (3 * (size_t)nblk + 4)));
         partial = dbuf;
         pw3 = dbuf + 3 * (size_t)nblk;
@@ -196,24 +194,51 @@ template <typename T> struct SolveT final : SolveImpl {
         return OPO_OK;
     }
 
-    int build_tables() {
+    // Entries (cplx<T>) of the table block: twiddles for NX and NT, the
+    // three D tables, GDD, the x and y factors of Q for three fields, the
+    // free-space x and y factors for s and i, and the modulator table, each
+    // entry a (hi, lo) pair.
+    static size_t table_entries(const opo_grid &g) {
+        return 2 * ((size_t)g.nx + g.nt + 3 * (size_t)g.nt + g.nt + 3 * (size_t)g.nx + 3 * (size_t)g.ny +
+                    2 * (size_t)g.nx + 2 * (size_t)g.ny + g.nt);
+    }
+    struct TableOffsets {
+        size_t twx, twt, D, G, qx, qy, fx, fy, E;
+    };
+    static TableOffsets table_offsets(const opo_grid &g) {
         const int nx = g.nx, ny = g.ny, nt = g.nt;
-        // Every entry is stored as a (hi, lo) pair: hi = value rounded to T,
-        // lo = rounding residual (zero for T = double).
-        const size_t total = 2 * ((size_t)nx + nt + 3 * nt + nt + 3 * nx + 3 * ny + 2 * nx + 2 * ny + nt);
-        std::vector<C> h(total);
         size_t off = 0;
         auto take = [&](size_t n) { size_t o = off; off += 2 * n; return o; };
-        const size_t o_twx = take(nx), o_twt = take(nt), o_T = take(3 * (size_t)nt), o_G = take(nt),
-                     o_qx = take(3 * (size_t)nx), o_qy = take(3 * (size_t)ny),
-                     o_fx = take(2 * (size_t)nx), o_fy = take(2 * (size_t)ny), o_E = take(nt);
+        TableOffsets o;
+        o.twx = take(nx);
+        o.twt = take(nt);
+        o.D = take(3 * (size_t)nt);
+        o.G = take(nt);
+        o.qx = take(3 * (size_t)nx);
+        o.qy = take(3 * (size_t)ny);
+        o.fx = take(2 * (size_t)nx);
+        o.fy = take(2 * (size_t)ny);
+        o.E = take(nt);
+        return o;
+    }
+
+    // Fill the host image h (table_entries(g) elements) of the table block
+    // from the parameters and the NT angular frequencies w.  With p.eom_on,
+    // tgrid (NT doubles of host scratch) receives the time grid.
+    static void fill_tables(const opo_grid &g, const opo_params &p, const double *w, C *h, double *tgrid) {
+        const int nx = g.nx, ny = g.ny, nt = g.nt;
+        const TableOffsets o = table_offsets(g);
+        const size_t o_twx = o.twx, o_twt = o.twt, o_T = o.D, o_G = o.G, o_qx = o.qx, o_qy = o.qy,
+                     o_fx = o.fx, o_fy = o.fy, o_E = o.E;
         // put(i, re, im): the pair of one entry at h[i] (hi) and h[i + 1] (lo);
-        // entry j of a table starting at o lives at i = o + 2 j.
+        // entry j of a table starting at o lives at i = o + 2 j.  hi = value
+        // rounded to T, lo = rounding residual (zero for T = double).
         auto put = [&](size_t i, double re, double im) {
             const T hr = (T)re, hi = (T)im;
             h[i] = C{hr, hi};
             h[i + 1] = C{(T)(re - (double)hr), (T)(im - (double)hi)};
         };
+        for (size_t i = 0; i < table_entries(g); ++i) h[i] = C{(T)0, (T)0};
         for (int j = 0; j < nx; ++j) {
             double c, s;
             unit_root_neg(j, nx, c, s);
@@ -224,7 +249,6 @@ template <typename T> struct SolveT final : SolveImpl {
             unit_root_neg(j, nt, c, s);
             put(o_twt + 2 * (size_t)j, c, s);
         }
-        const double *w = omega.data();
         // D: eiLz_X / NT (Solver3D.cuh:153-169; the 1/NT of the time pair folded in).
         for (int f = 0; f < 3; ++f) {
             const double att = std::exp(-0.5 * p.alpha[f] * p.dz) / nt;
@@ -276,30 +300,50 @@ template <typename T> struct SolveT final : SolveImpl {
         // Modulator on s and i: exp(i beta sin(2 pi fpm t_j)) on the time grid
         // t_j = -W/2 + j W/(NT-1) (KernelEOM, Solver1D.cuh:62-72).
         if (p.eom_on) {
-            std::vector<double> t(nt);
-            opo_time_grid(nt, p.t_window, t.data());
+            opo_time_grid(nt, p.t_window, tgrid);
             for (int k = 0; k < nt; ++k) {
                 double c, s2;
-                cis(p.eom_beta * std::sin(kTwoPi * std::fmod(p.eom_fpm * t[k], 1.0)), c, s2);
+                cis(p.eom_beta * std::sin(kTwoPi * std::fmod(p.eom_fpm * tgrid[k], 1.0)), c, s2);
                 put(o_E + 2 * (size_t)k, c, s2);
             }
         }
-        if (hipMalloc(&tables, total * sizeof(C)) != hipSuccess)
-            return fail(OPO_ERR_NOMEM, "hipMalloc(tables) failed");
-        CK(hipMemcpy(tables, h.data(), total * sizeof(C), hipMemcpyHostToDevice));
-        tw_x = tables + o_twx;
-        tw_t = tables + o_twt;
+    }
+
+    // Point the table pointers into a device table block.
+    void bind_tables(C *block) {
+        const TableOffsets o = table_offsets(g);
+        tables = block;
+        tw_x = tables + o.twx;
+        tw_t = tables + o.twt;
         for (int f = 0; f < 3; ++f) {
-            tabT[f] = tables + o_T + 2 * (size_t)f * nt;
-            qx[f] = tables + o_qx + 2 * (size_t)f * nx;
-            qy[f] = tables + o_qy + 2 * (size_t)f * ny;
+            tabT[f] = tables + o.D + 2 * (size_t)f * g.nt;
+            qx[f] = tables + o.qx + 2 * (size_t)f * g.nx;
+            qy[f] = tables + o.qy + 2 * (size_t)f * g.ny;
         }
-        tabG = tables + o_G;
-        tabE = p.eom_on ? tables + o_E : nullptr;
+        tabG = tables + o.G;
+        tabE = p.eom_on ? tables + o.E : nullptr;
         for (int f = 0; f < 2; ++f) {
-            qfsx[f] = tables + o_fx + 2 * (size_t)f * nx;
-            qfsy[f] = tables + o_fy + 2 * (size_t)f * ny;
+            qfsx[f] = tables + o.fx + 2 * (size_t)f * g.nx;
+            qfsy[f] = tables + o.fy + 2 * (size_t)f * g.ny;
         }
+    }
+
+    // Power partials: enough blocks to spread the sum, fixed per grid.
+    static int power_blocks(size_t cells) {
+        const size_t want = (cells + 256 * 16 - 1) / (256 * 16);
+        return (int)(want < 1 ? 1 : (want > 1024 ? 1024 : want));
+    }
+
+    int build_tables() {
+        const size_t total = table_entries(g);
+        std::vector<C> h(total);
+        std::vector<double> tg(g.nt);
+        fill_tables(g, p, omega.data(), h.data(), tg.data());
+        C *block = nullptr;
+        if (hipMalloc(&block, total * sizeof(C)) != hipSuccess)
+            return fail(OPO_ERR_NOMEM, "hipMalloc(tables) failed");
+        bind_tables(block);
+        CK(hipMemcpy(tables, h.data(), total * sizeof(C), hipMemcpyHostToDevice));
         return OPO_OK;
     }
 
@@ -408,12 +452,6 @@ template <typename T> struct SolveT final : SolveImpl {
         a.tab_stride = tab_stride;
         a.first_field = first;
         a.rk = rk;
-        if (g.nx > 1 && !rk.has_phase) {
-            const int tx = tune_tile_px(sizeof(T) == 8, g.nt, tune_get(sizeof(T) == 8, 0, g.nt));
-            if (tx == 0 || g.nx % tx != 0)
-                return fail(OPO_ERR_UNSUPPORTED, "tuned tile of %d pixels does not divide NX=%d (NT=%d)", tx,
-                            g.nx, g.nt);
-        }
         PROF(stage, (launch_tline3<T>(stream, geo(), a, tw_t, st)));
         return OPO_OK;
     }
@@ -566,7 +604,7 @@ template <typename T> struct SolveT final : SolveImpl {
         if (!use_graphs) return enqueue();
         // the launch variants are process-wide: re-capture when they changed
         const bool fp64 = sizeof(T) == 8;
-        const int cur[2] = {tune_get(fp64, 0, g.nt), tune_get(fp64, 1, g.nx)};
+        const int cur[2] = {time_variant_for(fp64, g.nt, g.nx), tune_get(fp64, 1, g.nx)};
         if (cur[0] != graph_variants[0] || cur[1] != graph_variants[1]) {
             for (auto &gx : graph)
                 if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
@@ -915,10 +953,20 @@ struct opo_batch {
     bool dirty = true;
     std::vector<int> built_variants;  // tuned launch variants the graph uses
 
+    // Variant of the fused pass over NT class nt: the tuned one when its
+    // tile divides the NX of every resident solve of the class, else
+    // variant 0 (its tile divides every supported NX).
+    int class_time_variant(int nt) const {
+        const bool fp64 = prec == OPO_FP64;
+        const int v = tune_get(fp64, 0, nt);
+        for (const BatchEntry *e : resident)
+            if (e->g.nt == nt && time_variant_for(fp64, nt, e->g.nx) != v) return 0;
+        return v;
+    }
     std::vector<int> current_variants() const {
         std::vector<int> v;
         const bool fp64 = prec == OPO_FP64;
-        for (const Range &r : trange) v.push_back(tune_get(fp64, 0, r.cls));
+        for (const Range &r : trange) v.push_back(class_time_variant(r.cls));
         for (const Range &r : xyrange) v.push_back(tune_get(fp64, 1, r.cls));
         return v;
     }
@@ -985,7 +1033,7 @@ struct opo_batch {
         // launches captured below both follow it, whatever the process-wide
         // choice becomes meanwhile (a change makes the next step rebuild).
         for (auto &kv : by_nt) {
-            Range r{kv.first, (int)work.size(), 0, tune_get(fp64, 0, kv.first)};
+            Range r{kv.first, (int)work.size(), 0, class_time_variant(kv.first)};
             const int tx = tune_tile_px(fp64, kv.first, r.v);
             for (int i : kv.second) {
                 const opo_grid &g = resident[i]->g;
@@ -1123,6 +1171,9 @@ int opo_single_pass(opo_solve *s) { SOLVE_OR_FAIL(s); return s->s->ssfm(); }
 int opo_run_round_trips(opo_solve *s, int n, double *power) {
     SOLVE_OR_FAIL(s);
     if (n < 0) return fail(OPO_ERR_ARG, "n must be >= 0");
+    // A round trip of the reference runs its slice loop at least once
+    // (Solver3D.cuh:949, z = 0 <= Lcr) before the feedback.
+    if (s->s->p.n_slices < 1) return fail(OPO_ERR_ARG, "round trips need n_slices >= 1");
     if (n == 0) return OPO_OK;
     int rc = s->s->enqueue_trips(n);
     if (rc) return rc;
@@ -1150,6 +1201,7 @@ int opo_courant_ok(const opo_params *p) {
 int opo_profile_trips(opo_solve *s, int n, double ms[OPO_N_STAGES], int launches[OPO_N_STAGES]) {
     SOLVE_OR_FAIL(s);
     if (n < 1 || !ms || !launches) return fail(OPO_ERR_ARG, "opo_profile_trips: bad argument");
+    if (s->s->p.n_slices < 1) return fail(OPO_ERR_ARG, "round trips need n_slices >= 1");
     return s->s->profile_trips(n, ms, launches);
 }
 int opo_autotune(opo_solve *s, int reps, char *report, size_t report_len) {
@@ -1178,13 +1230,23 @@ int opo_sync(opo_solve *s) {
 }
 
 // ------------------------------------------------------------- batching
-size_t opo_batch_footprint(const opo_grid *g, int precision) {
-    if (!g) return 0;
+size_t opo_batch_footprint(const opo_grid *g, int precision, int n_trips) {
+    if (!g || n_trips < 0) return 0;
     const size_t n = (size_t)g->nx * g->ny * g->nt;
     const size_t cb = precision == OPO_FP64 ? 16 : 8;
-    // Api, Ap, As, Ai + tables (hi/lo pairs) + small per-solve buffers
-    const size_t tabs = 2 * (size_t)(6 * g->nx + 5 * g->ny + 5 * g->nt) * cb;
-    return 4 * n * cb + tabs + 64 * 1024;
+    // Every device allocation of a resident solve, each rounded up to the
+    // allocation granularity assumed here: Api, Ap, As, Ai; the table block
+    // (twiddles, D, GDD, Q, free-space and modulator tables as hi/lo pairs);
+    // the power partials (one per 4096 cells, 1 to 1024, three fields) and
+    // counters; the per-trip power series of the batch.
+    const size_t kGran = 64 * 1024;
+    auto up = [&](size_t b) { return (b + kGran - 1) / kGran * kGran; };
+    const size_t tabs = 2 * (size_t)(6 * g->nx + 5 * g->ny + 6 * g->nt) * cb;
+    size_t nblk = (n + 4095) / 4096;
+    nblk = nblk < 1 ? 1 : (nblk > 1024 ? 1024 : nblk);
+    const size_t dbuf = sizeof(double) * (3 * nblk + 4);
+    const size_t series = sizeof(double) * 3 * (size_t)(n_trips > 0 ? n_trips : 1);
+    return 4 * up(n * cb) + up(tabs) + up(dbuf) + up(4 * sizeof(int)) + up(series);
 }
 
 int opo_batch_create(int device, size_t budget, int precision, opo_batch **out) {
@@ -1213,6 +1275,7 @@ int opo_batch_submit(opo_batch *b, const opo_grid *g, const opo_params *p, const
         return fail(OPO_ERR_ARG, "opo_batch_submit: bad argument");
     int rc = validate(g, p);
     if (rc) return rc;
+    if (p->n_slices < 1) return fail(OPO_ERR_ARG, "round trips need n_slices >= 1");
     if (p->precision != b->prec) return fail(OPO_ERR_ARG, "precision differs from the batch's");
     if (!batch_supported(g->nx, g->ny, g->nt, p->precision == OPO_FP64))
         return fail(OPO_ERR_UNSUPPORTED, "grid %dx%dx%d is not supported in a batch", g->nx, g->ny, g->nt);
@@ -1220,7 +1283,7 @@ int opo_batch_submit(opo_batch *b, const opo_grid *g, const opo_params *p, const
         return fail(OPO_ERR_UNSUPPORTED, "a batch runs solves without the dk phase term");
     if (p->no_diffraction)
         return fail(OPO_ERR_UNSUPPORTED, "a batch runs solves with diffraction (no plane-wave model)");
-    const size_t fp = opo_batch_footprint(g, p->precision);
+    const size_t fp = opo_batch_footprint(g, p->precision, n_trips);
     if (fp > b->budget) return fail(OPO_ERR_BUDGET, "solve needs %zu bytes, budget is %zu", fp, b->budget);
     std::lock_guard<std::mutex> lk(b->mu);
     if (b->n_slices >= 0 && p->n_slices != b->n_slices)

@@ -275,6 +275,12 @@ int tune_tile_px(bool fp64, int nt, int v) {
     return ok ? tx : 0;
 }
 
+int time_variant_for(bool fp64, int nt, int nx) {
+    const int v = tune_get(fp64, 0, nt);
+    const int tx = tune_tile_px(fp64, nt, v);
+    return (tx > 0 && nx % tx == 0) ? v : 0;
+}
+
 int tune_per_cu(bool fp64, int axis, int n, int v) {
     if (axis == 1) return xy_variant_per_cu(n, v);
     int tx = 0, bs = 0, per_cu = 0;
@@ -330,7 +336,7 @@ static hipError_t tline3_dispatch(hipStream_t st, const Geometry &g, const TLine
             hipLaunchKernelGGL((k_tline3<T, NT, 1, 256, false>), grid, dim3(256), 0, st, a, g.nx, g.ny, tw, status);
         return hipGetLastError();
     }
-    const int v = a.rk.has_phase ? 0 : tune_get(sizeof(T) == 8, 0, NT);
+    const int v = a.rk.has_phase ? 0 : time_variant_for(sizeof(T) == 8, NT, g.nx);
     hipError_t e = hipErrorInvalidValue;
     static_for<0, kTimeVariants>([&](auto vv) {
         constexpr int V = decltype(vv)::value;

@@ -97,6 +97,12 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v);
 int tune_get(bool fp64, int axis, int n);
 bool tune_set(bool fp64, int axis, int n, int v);
 int tune_tile_px(bool fp64, int nt, int v);   // pixels per tile of a t-pass variant
+// The t-pass variant a launch over pixel rows of nx pixels uses: the tuned
+// choice for NT when its tile divides nx, else variant 0, whose tile (16,
+// 8, 4, 2 or 1 pixels) divides every supported NX.  The choice is
+// process-wide and keyed by NT alone, so a tile tuned on a wide grid may
+// not fit a narrow one.
+int time_variant_for(bool fp64, int nt, int nx);
 int tune_block(bool fp64, int axis, int n, int v);
 int tune_per_cu(bool fp64, int axis, int n, int v);
 // Grid of a persistent launch: per_cu workgroups per CU of the current

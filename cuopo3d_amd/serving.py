@@ -87,15 +87,29 @@ def serve(requests, budget_bytes, precision=OPO_FP32, keep_fields=True, clock=ti
         now = clock() - t0
         while i < len(pending) and pending[i].arrival <= now:
             r = pending[i]
-            r.ticket = batch.submit(r.grid, r.params, *r.fields, r.n_trips)
-            outstanding[r.ticket] = r
             i += 1
+            try:
+                r.ticket = batch.submit(r.grid, r.params, *r.fields, r.n_trips)
+            except RuntimeError as e:   # rejected (grid, budget, parameters): this request only
+                r.error = str(e)
+                r.done_at = clock() - t0
+                continue
+            outstanding[r.ticket] = r
         res, queued, _ = batch.stats()
         if res == 0 and queued == 0:
             if i < len(pending):   # idle until the next arrival
                 time.sleep(max(0.0, min(0.01, pending[i].arrival - (clock() - t0))))
             continue
-        batch.step()
+        try:
+            batch.step()
+        except RuntimeError as e:
+            # the batch itself failed: every request not yet finished fails with it
+            for r in list(outstanding.values()) + pending[i:]:
+                r.error = f"batch step failed: {e}"
+                r.done_at = clock() - t0
+            outstanding.clear()
+            i = len(pending)
+            break
         steps += 1
         max_res = max(max_res, batch.stats()[0])
         for t in list(outstanding):

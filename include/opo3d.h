@@ -47,7 +47,7 @@ extern "C" {
 #define OPO_FP64 1   /* all-FP64 device path                                 */
 
 /* Supported axis lengths of the device kernels.  nx, ny in {1, 16, 32, 64,
- * 96, 128}; nt in {1, 16, 32, 64, 128, 256, 512, 1024}.  A length of 1 turns
+ * 96, 128}; nt in {1, 16, 32, 64, 128, 256, 512, 1024, 2048}.  A length of 1 turns
  * the axis off (NX = NY = 1 is the reference's 1D plane-wave build,
  * src/headers/Solver1D.cuh; NT = 1 its CW build, DtypesConsts.cuh:38). */
 
@@ -180,6 +180,7 @@ int opo_single_pass(opo_solve *s);
  * opo_trips_done() then tells how many trips completed. */
 int opo_run_round_trips(opo_solve *s, int n, double *power);
 int opo_trips_done(const opo_solve *s);
+
 /* averagePowerCutXY(A) for p, s, i (Solver3D.cuh:529-545), compensated. */
 int opo_power(opo_solve *s, double out[3]);
 /* averagePowerTime(A, x, y) (Solver3D.cuh:509-523): *out = prefactor *
@@ -256,8 +257,10 @@ int opo_batch_step(opo_batch *b, int *n_active);
  * device memory is released. */
 int opo_batch_poll(opo_batch *b, uint64_t ticket, int *done, double *power,
                    void *ap, void *as, void *ai);
-/* Footprint in bytes a solve of this grid/precision occupies in a batch. */
-size_t opo_batch_footprint(const opo_grid *grid, int precision);
+/* Device bytes a solve of this grid/precision run for n_trips round trips
+ * occupies in a batch: its four fields, tables, power partials, counters
+ * and power series, each allocation rounded up to 64 KiB. */
+size_t opo_batch_footprint(const opo_grid *grid, int precision, int n_trips);
 /* Counts: resident solves, queued solves, bytes in use. */
 int opo_batch_stats(opo_batch *b, int *resident, int *queued,
                     size_t *bytes_used);

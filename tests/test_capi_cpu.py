@@ -74,8 +74,15 @@ def test_null_handles_are_errors():
 def test_batch_footprint():
     lib = capi.load()
     g = Grid(128, 128, 1024)
-    fp = lib.opo_batch_footprint(C.byref(g), 0)
+    fp = lib.opo_batch_footprint(C.byref(g), 0, 300)
     assert 4 * 8 * 128 * 128 * 1024 <= fp < 4 * 8 * 128 * 128 * 1024 * 1.01
+    # tables (6 NX + 5 NY + 6 NT entries, hi/lo pairs), partials, counters and
+    # the 3-double power series of every trip each take at least one 64 KiB unit
+    small = Grid(16, 16, 16)
+    f1 = lib.opo_batch_footprint(C.byref(small), 0, 1)
+    assert f1 == 4 * 65536 + 4 * 65536
+    f10k = lib.opo_batch_footprint(C.byref(small), 0, 10000)
+    assert f10k - f1 == 3 * 65536   # 240,000 B of series: four units instead of one
 
 
 def test_tune_set_rejects_missing_variants():
@@ -88,3 +95,4 @@ def test_tune_set_rejects_missing_variants():
     assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 48, 0) != capi.OPO_OK    # no such NX
     assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, 256, keep) == capi.OPO_OK
     assert lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, 256) == keep
+

@@ -219,3 +219,38 @@ def test_fp64_variants_give_the_same_fields(n, require_gpu):
     finally:
         for a, v in keep.items():
             lib.opo_tune_set(OPO_FP64, a, nt if a == OPO_TUNE_TIME_PASS else nx, v)
+
+
+def test_wide_tuned_tile_falls_back_on_narrow_grids(restore_tuning, require_gpu):
+    """The t-pass choice is keyed by NT only: a 32-pixel tile tuned on a wide
+    grid does not divide a 16-pixel row.  Single solves and a resident batch
+    of 16 x 16 x 128 solves then run the default tile (variant 0) and give
+    its fields bitwise; a 64-pixel grid in the same batch class keeps
+    working too."""
+    lib = restore_tuning
+    draws = configs.sweep_draws(n_solves=3, seed=13)
+    solves = []
+    for d, nxy in zip(draws, (16, 16, 64)):
+        d = dict(d)
+        d["nxy"], d["nt"] = nxy, 128
+        solves.append(configs.sweep_solve(d, n_round_trips=2))
+    assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, 128, 0) == OPO_OK
+    ref = [_run(g, p, f, trips=2) for (g, p, ph, f) in solves]
+    ref_batch, _ = batch_run(solves, 2)
+    # variants 6-8 of a 128-sample line: tiles of twice the default 16 pixels
+    wide = [v for v in (7, 6, 8) if v in _valid(lib, OPO_TUNE_TIME_PASS, 128)]
+    assert wide, "no 32-pixel variant for NT=128"
+    assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, 128, wide[0]) == OPO_OK
+    for (g, p, ph, f), (rf, rp) in zip(solves, ref):
+        got, pw = _run(g, p, f, trips=2)
+        for a, b in zip(got, rf):
+            assert np.array_equal(a, b), (g.nx, "single solve")
+        assert np.array_equal(pw, rp)
+    got_batch, _ = batch_run(solves, 2)
+    for (pg, fg), (pr, fr), (g, _, _, _) in zip(got_batch, ref_batch, solves):
+        for a, c in zip(fg, fr):
+            assert np.array_equal(a, c), (g.nx, "batch")
+        assert np.array_equal(pg, pr)
+    for (pg, fg), (rf, rp) in zip(got_batch, ref):
+        for a, c in zip(fg, rf):
+            assert np.array_equal(a, c)
