@@ -1,22 +1,26 @@
 """Benchmark of the split-step path on one MI355X.
 
-Workload (one "step" = one round trip): the singly-resonant cavity solve,
-64x64x256 complex64 fields, MgO:PPLN 5 mm, NZ = 100 (100 slices per trip),
-Rs = 0.98, idler not resonant, GDD compensation, free-space feedback; the
-fields start from a focused 2 ps pump and seeded noise and stay resident in
-HBM for the whole timed region.  value = cell-steps/s (cells x slices x
-round trips / wall time).
+Headline workload (one "step" = one round trip of every solve): the batched
+parameter sweep, 64 singly-resonant cavity solves of the seed-3 draw (grids
+32x32x128 ... 128x128x512, complex64, MgO:PPLN 5 mm, NZ = 100 so 100
+slices per trip; pump power, waist, detuning and mirror reflectivity drawn
+per solve), all resident in HBM in one batch, one launch per split-step
+stage and size class, the round trip captured as a HIP graph.  It replaces
+the reference's one-process-per-point sweep (src/cuOPO3D.sh:56-86).
+value = cell-steps/s (cells x slices x trips of all 64 solves / wall time).
 
-The roofline entry is for the dominant kernel, timed with HIP events on the
-solve's stream (opo_profile_trips) over a second run of the same trips; its
-algorithmic bytes are 16 B per cell per field touched (8 read + 8 written).
-The CPU baseline is the FP64 oracle on the host cores over the first round
-trips of the same workload.
+Kernel times come from HIP events recorded inside the replayed trip graph
+around every launch, over the timed region (opo_batch_profile); the
+roofline entry is the kernel with the most time, its algorithmic bytes
+16 B per cell per field read and written over the cells of its launch.
+The CPU baseline is the FP64 oracle on the host cores over the smallest and
+largest grids of the same sweep, 2 round trips each.
 """
 import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -28,45 +32,50 @@ import numpy as np  # noqa: E402
 METRIC = "cell-steps/s & %HBM roofline; p50/p99 latency of batched OPO sweeps"
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12   # FP32 VALU lane operations, no packed math
-# fields touched per launch of each stage kind
+# fields touched per launch of each stage kind (single-solve profiling mode)
 STAGE_FIELDS = {"nonlinear": 3, "time_op": 3, "xy_op": 3, "free_space": 2, "gdd": 2, "power": 3}
 STAGE_RW = {"power": 1}  # the power reduction only reads
+# kernel of each batched stage kind
+BATCH_KERNEL = {"time_op": "kb_time", "gdd": "kb_time (trip end)", "xy_op": "kb_xy",
+                "free_space": "kb_xy (free space)", "power": "kb_power"}
+NT_LIST = (16, 32, 64, 128, 256, 512, 1024, 2048)
+NX_LIST = (16, 32, 64, 96, 128)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-trips", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20, help="timed round trips of the batched sweep")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the batched sweep (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-trips", type=int, default=20)
     ap.add_argument("--no-plain", action="store_true", help="skip the rocFFT comparator")
-    ap.add_argument("--sweep-trips", type=int, default=10, help="0 skips the batched sweep")
-    ap.add_argument("--serve-requests", type=int, default=16, help="0 skips the serving run")
-    ap.add_argument("--serve-rate", type=float, default=10.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
-    ap.add_argument("--tune-reps", type=int, default=5, help="timed stretches per tuning candidate")
+    ap.add_argument("--tune-reps", type=int, default=3, help="timed stretches per tuning candidate")
+    ap.add_argument("--variants-out", help="write the launch variants in use to this JSON file")
+    ap.add_argument("--variants-in", help="set the launch variants from this JSON file (no tuning)")
+    ap.add_argument("--serve-overload", action="store_true",
+                    help="also serve 16 requests at 10/s with 50-300 trips (about 50 s)")
     ap.add_argument("--horizon-trips", type=int, default=0,
                     help="also compare FP32 and FP64 device fields with the FP64 oracle over this "
-                         "many round trips (checkpoints 1, 3, 10, 30, 100, 200, ...); off by default")
-    ap.add_argument("--grid", type=int, nargs=2, default=(64, 256), metavar=("NXY", "NT"),
-                    help="grid of the cavity workload (default 64 256); other grids are for "
-                         "profiling and skip the CPU and comparator legs")
-    ap.add_argument("--no-reference-grid", action="store_true",
-                    help="skip the run on the reference's compiled-default 128x128x2048 grid")
+                         "many round trips of the 64x64x256 cavity (checkpoints); off by default")
+    ap.add_argument("--cavity-grid", type=int, nargs=2, metavar=("NXY", "NT"),
+                    help="instead of the sweep, time one cavity solve on this grid (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 128 MiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs one solve per GPU on one GPU; --gpus {args.gpus} is not "
-              "supported (independent solves shard across GPUs by footprint, no collectives).",
-              file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent solves shard across GPUs by footprint, no collectives).", file=sys.stderr)
         sys.exit(2)
 
     import cuopo3d_amd as co
     from cuopo3d_amd import configs
-    from cuopo3d_amd.capi import STAGES
 
     lib = co.load()
     if lib.opo_device_count() < 1:
@@ -81,154 +90,358 @@ def main():
         cs.set_fields(np.zeros((1024, 128, 128), np.complex64), None, None)
         cs.sync()
         cs.close()
-    default_grid = tuple(args.grid) == (64, 256)
-    if not default_grid:
-        args.no_cpu = args.no_plain = True
-        args.sweep_trips = args.serve_requests = 0
-    n = (args.grid[0], args.grid[0], args.grid[1])
-    grid, p, ph, fields = configs.cavity_64(n=n, n_round_trips=args.warmup + args.steps)
-    cells = grid.nx * grid.ny * grid.nt
-    S = p.n_slices
-    s = co.Solver(grid, p, device=0)
-    s.set_fields(*fields)
-    tune_report = s.autotune(reps=args.tune_reps) if not args.no_tune else "not tuned"
-    s.runOPO(args.warmup)
-    s.sync()   # all of the solve's work is on its stream
+    if args.cavity_grid:
+        print(json.dumps(cavity_profile_mode(co, configs, lib, args)))
+        return
+
+    out = sweep_headline(co, configs, lib, args)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_leg(co, configs)
+    if not args.headline_only:
+        out["cavity_64x64x256"] = cavity_leg(co, configs, args)
+        out["single_pass"] = single_pass_leg(co, configs)
+        out["small_grid_sweep"] = small_sweep_leg(co, configs)
+        out["one_at_a_time"] = one_at_a_time_leg(co, configs, out["value"])
+        out["reference_default_grid"] = reference_grid_leg(co, configs, args.tune_reps)
+        if not args.no_plain:
+            out["plain_rocfft"] = plain_leg(out["one_at_a_time"], out["value"])
+        out["serving"] = serving_leg(co, configs, args.serve_overload)
+        if args.horizon_trips > 0:
+            out["horizon"] = horizon_leg(co, configs, args.horizon_trips)
+    print(json.dumps(out))
+
+
+# ------------------------------------------------------------ variants
+def variants_get(lib, prec=0):
+    from cuopo3d_amd.capi import OPO_TUNE_TIME_PASS, OPO_TUNE_XY_PASS
+    return {"t": {str(n): lib.opo_tune_get(prec, OPO_TUNE_TIME_PASS, n) for n in NT_LIST},
+            "xy": {str(n): lib.opo_tune_get(prec, OPO_TUNE_XY_PASS, n) for n in NX_LIST}}
+
+
+def variants_set(lib, v, prec=0):
+    from cuopo3d_amd.capi import OPO_TUNE_TIME_PASS, OPO_TUNE_XY_PASS, OPO_OK
+    for axis, key in ((OPO_TUNE_TIME_PASS, "t"), (OPO_TUNE_XY_PASS, "xy")):
+        for n, k in v[key].items():
+            if lib.opo_tune_set(prec, axis, int(n), int(k)) != OPO_OK:
+                raise RuntimeError(f"variant {k} does not exist for {key} {n}")
+
+
+# ---------------------------------------------------------- headline
+def sweep_headline(co, configs, lib, args, n_solves=64, trips=100):
+    """The batched 64-solve sweep: value, per-kernel event times inside the
+    replayed graph, roofline of the dominant kernel."""
+    draws = configs.sweep_draws(n_solves=n_solves, seed=3)
+    solves = [configs.sweep_solve(d, n_round_trips=trips) for d in draws]
+    grids = sorted({(g.nx, g.ny, g.nt) for (g, _, _, _) in solves})
+    tune = []
+    if args.variants_in:
+        variants_set(lib, json.load(open(args.variants_in)))
+        tune = ["variants set from " + os.path.basename(args.variants_in)]
+    elif not args.no_tune:
+        t0 = time.perf_counter()
+        for rep in co.autotune_grids(grids, reps=args.tune_reps):
+            tune += [ln.strip() for ln in rep.strip().splitlines()]
+        log(f"sweep autotune: {time.perf_counter() - t0:.1f} s")
+    variants = variants_get(lib)
+    if args.variants_out:
+        with open(args.variants_out, "w") as fh:
+            json.dump(variants, fh)
+    work = sum(g.nx * g.ny * g.nt * p.n_slices for (g, p, _, _) in solves)   # cell-steps per trip
+    b = co.Batch(200 << 30)
+    tickets = [b.submit(g, p, *f, trips) for (g, p, _, f) in solves]
+    assert args.warmup + args.steps + 4 <= trips, "the timed trips must fit the solves' 100 trips"
     t0 = time.perf_counter()
-    pw = s.runOPO(args.steps)
-    s.sync()
-    t1 = time.perf_counter()
-    dt = t1 - t0
-    value = cells * S * args.steps / dt
-
-    # per-stage kernel time with HIP events on the solve's stream
-    ms = (C.c_double * 6)()
-    nl = (C.c_int * 6)()
-    co.check(lib.opo_profile_trips(s.h, args.profile_trips, ms, nl), "opo_profile_trips")
-    stages = {}
-    for k, name in enumerate(STAGES):
-        if nl[k] == 0:
+    b.step()   # admission, descriptors, graph capture, first trip
+    admit_s = time.perf_counter() - t0
+    for _ in range(max(0, args.warmup - 1)):
+        b.step()
+    # a few trips without events: the rate the events are checked against
+    t0 = time.perf_counter()
+    for _ in range(3):
+        b.step()
+    plain_rate = work * 3 / (time.perf_counter() - t0)
+    b.profile(True)
+    b.step()   # re-capture with the events (not timed)
+    b.profile(True)   # sums back to zero
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.step()
+    dt = time.perf_counter() - t0
+    value = work * args.steps / dt
+    prof = b.profile_read()
+    resident = b.stats()[0]
+    b.close()
+    assert resident == n_solves
+    stages = []
+    for e in prof:
+        if e["launches"] == 0:
             continue
-        avg_ms = ms[k] / nl[k]
-        alg = cells * STAGE_FIELDS[name] * 8 * (STAGE_RW.get(name, 2))
-        stages[name] = {"launches": nl[k], "avg_us": avg_ms * 1e3, "total_ms": ms[k],
-                        "alg_bytes": alg, "GBs": alg / (avg_ms * 1e-3) / 1e9}
-    dom = max(stages, key=lambda k: stages[k]["total_ms"])
-    d = stages[dom]
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["GBs"], 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(d["GBs"] / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(grid, dom)}
-    gpu_kernel_ms = sum(v["total_ms"] for v in stages.values()) / args.profile_trips
-
-    out = {
+        avg_s = e["ms"] / e["launches"] * 1e-3
+        stages.append({"kernel": BATCH_KERNEL[e["stage"]], "stage": e["stage"], "class": e["size"],
+                       "launches": e["launches"], "avg_us": round(avg_s * 1e6, 2),
+                       "total_ms": round(e["ms"], 3), "alg_bytes": int(e["alg_bytes"]),
+                       "GBs": round(e["alg_bytes"] / avg_s / 1e9, 1)})
+    stages.sort(key=lambda x: -x["total_ms"])
+    dom = stages[0]
+    key = f"{dom['stage']}/{dom['class']}"
+    roofline = {"bound": "hbm", "kernel": f"{dom['kernel']} {'NT' if dom['stage'] in ('time_op', 'gdd') else 'NX'}"
+                                          f"={dom['class']}",
+                "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["GBs"] / HBM_PEAK_GBS, 4),
+                "traffic": counter_summary("counters_sweep_64.md", "traffic_bytes_per_launch", key)}
+    kernel_ms_trip = sum(s["total_ms"] for s in stages) / args.steps
+    cls_cells = {}
+    for (g, p, _, _) in solves:
+        cls_cells.setdefault(f"{g.nx}x{g.ny}x{g.nt}", 0)
+        cls_cells[f"{g.nx}x{g.ny}x{g.nt}"] += 1
+    return {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64 (f32)",
-        "data": "synthetic (focused Gaussian pump + seeded Philox noise)",
-        "config": {"workload": f"singly-resonant cavity {grid.nx}x{grid.ny}x{grid.nt}, MgO:PPLN 5 mm, "
-                               "100 slices/trip, window = round-trip time, Rs=0.98, idler not resonant "
-                               "(multiplier 1), GDD, free space; step = one round trip",
-                   "grid": [grid.nx, grid.ny, grid.nt], "slices": S},
+        "data": "synthetic (focused Gaussian pump + seeded Philox noise, parameters drawn from seed 3)",
+        "config": {"workload": "batched parameter sweep: 64 singly-resonant cavity solves of the seed-3 "
+                               "draw, grids 32x32x128 .. 128x128x512, MgO:PPLN 5 mm, 100 slices per trip, "
+                               "P 0.5-5 W, w0 30-80 um, detuning -pi..pi, R 0.9-0.99; one launch per "
+                               "stage and size class, trip replayed as a HIP graph; step = one round "
+                               "trip of all 64 solves",
+                   "solves": n_solves, "grids": cls_cells, "cell_steps_per_trip": work},
         "roofline": roofline,
-        "valu": valu_utilisation(grid, dom, d["avg_us"] * 1e-6),
-        "stages": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                   for k, v in stages.items()},
-        "gpu_kernel_ms_per_trip": gpu_kernel_ms,
-        # The same rate priced in the 5-pass accounting of a split-step step
-        # (RK4 pair, forward xy, forward t with the propagator, inverse t,
-        # inverse xy: 5 x 3 fields x 16 B = 240 B per cell-step): the HBM
-        # rate an unfused schedule would need to match this throughput.
-        "five_pass_equivalent": {"bytes_per_cell_step": 240, "GBs": round(value * 240 / 1e9, 1),
-                                 "frac_of_peak": round(value * 240 / 1e9 / HBM_PEAK_GBS, 4)},
-        "autotune": tune_report.strip().splitlines(),
-        "signal_power_last_trip_W": float(pw[-1, 1]),
+        "valu": valu_from_counters("counters_sweep_64.md", key, dom["alg_bytes"] / 48.0,
+                                   dom["avg_us"] * 1e-6),
+        "kernels": stages,
+        "kernel_ms_per_step": round(kernel_ms_trip, 3),
+        "rate_without_events": plain_rate,
+        "admission_s": round(admit_s, 3),
+        "variants": variants,
+        "autotune": tune,
     }
 
-    single_fields = {}
-    if default_grid:
-        out["single_pass"], single_fields = single_pass_leg(co, configs)
-        out["idler_dumped"] = idler_dumped_leg(co, configs, args.steps)
-        out["fp64_path"] = fp64_leg(co, configs, min(args.steps, 50))
-        if not args.no_reference_grid:
-            out["reference_default_grid"] = reference_grid_leg(co, configs, args.tune_reps)
-    if not args.no_cpu:
-        from oracle.pyoracle import Oracle, get_threads
-        g2, p2, _, f2 = configs.cavity_64(n_round_trips=args.cpu_trips)
-        o = Oracle(g2, p2)
-        o.set_fields(*f2)
+
+def counter_summary(fname, key, entry):
+    """Integer figure `key` of kernel entry `entry` (e.g. time_op/512) from a
+    committed counter summary under profiles/, or None."""
+    try:
+        text = open(os.path.join(ROOT, "profiles", fname)).read()
+    except OSError:
+        return None
+    m = re.search(re.escape(key) + r":.*?\b" + re.escape(entry) + r"=(\d+)", text)
+    return int(m.group(1)) if m else None
+
+
+def valu_from_counters(fname, entry, cells, avg_s):
+    """The kernels are bound by FP32 vector arithmetic as much as by HBM:
+    VALU lane operations per cell (rocprofv3 SQ_INSTS_VALU x 64 / cells,
+    from the committed counter summary) times the cells of a launch, over
+    its mean duration, against 256 CUs x 64 lanes x 2.4 GHz (FP32 without
+    packed math; an FMA counts once)."""
+    ops = counter_summary(fname, "valu_lane_ops_per_cell", entry)
+    if ops is None or avg_s <= 0:
+        return None
+    achieved = ops * cells / avg_s
+    return {"kernel": entry, "lane_ops_per_cell": ops, "achieved": round(achieved / 1e12, 2),
+            "peak": VALU_PEAK_T, "unit": "T lane-ops/s", "frac": round(achieved / 1e12 / VALU_PEAK_T, 4)}
+
+
+# ------------------------------------------------------------- CPU leg
+def cpu_leg(co, configs):
+    """FP64 oracle on the host cores: the smallest and largest grids of the
+    same 64-solve draw, 2 round trips each, with the GPU's error on them."""
+    from oracle.pyoracle import Oracle, get_threads
+    solves = [configs.sweep_solve(d, n_round_trips=2) for d in configs.sweep_draws(n_solves=64, seed=3)]
+    cells_of = [g.nx * g.ny * g.nt for (g, _, _, _) in solves]
+    work, secs, parts = 0.0, 0.0, {}
+    for tag, k in (("smallest", int(np.argmin(cells_of))), ("largest", int(np.argmax(cells_of)))):
+        g, p, _, f = solves[k]
+        o = Oracle(g, p)
+        o.set_fields(*f)
         c0 = time.perf_counter()
-        pr = o.run_round_trips(args.cpu_trips)
+        pr = o.run_round_trips(2)
         c1 = time.perf_counter()
-        cpu_rate = cells * S * args.cpu_trips / (c1 - c0)
-        # parity of the same trips on the GPU
-        s2 = co.Solver(g2, p2, device=0)
-        s2.set_fields(*f2)
-        pg = s2.runOPO(args.cpu_trips)
-        gf = s2.get_fields()
-        rf = o.get_fields()
-        errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(gf, rf)]
-        perr = float(np.max(np.abs(pg[:, 1] - pr[:, 1]) / np.abs(pr[:, 1])))
-        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-steps/s", "cores": get_threads(),
-                               "kind": "port",
-                               "sample": f"FP64 oracle, first {args.cpu_trips} round trips of the "
-                                         f"same 64x64x256 cavity workload ({c1 - c0:.1f} s)"}
-        out["parity_vs_oracle"] = {"trips": args.cpu_trips, "field_rel_l2_p_s_i": errs,
-                                   "signal_power_rel_max": perr}
-        # the single-pass configs in full on the host, with the GPU's error; the
-        # oracle starts from the FP64 set-up (the FP32 set-up is its rounding)
-        from cuopo3d_amd.capi import OPO_FP64
-        others = {}
-        for (name, prec), gf in single_fields.items():
-            if prec:
-                continue
-            make = configs.single_pass_32 if name.startswith("single") else configs.femto_64
-            g3, p3, _, f3 = make(precision=OPO_FP64)
-            o3 = Oracle(g3, p3)
-            o3.set_fields(*f3)
-            c2 = time.perf_counter()
-            o3.ssfm()
-            c3 = time.perf_counter()
-            rf3 = o3.get_fields()
-            entry = {"value": g3.nx * g3.ny * g3.nt * p3.n_slices / (c3 - c2), "unit": "cell-steps/s",
-                     "seconds": c3 - c2, "cores": get_threads()}
-            for (n2, q), fl in single_fields.items():
-                if n2 == name:
-                    entry["fp64_rel_l2_p_s_i" if q else "fp32_rel_l2_p_s_i"] = [
-                        float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fl, rf3)]
-            others[name] = entry
-        out["cpu_baseline"]["single_pass_configs"] = others
-        # the smallest and largest grids of the sweep, 2 round trips each
-        sweep_cpu = {}
-        solves = [configs.sweep_solve(d, n_round_trips=2) for d in configs.sweep_draws(n_solves=64, seed=3)]
-        cells_of = [g.nx * g.ny * g.nt for (g, _, _, _) in solves]
-        for tag, k in (("smallest", int(np.argmin(cells_of))), ("largest", int(np.argmax(cells_of)))):
-            g4, p4, _, f4 = solves[k]
-            o4 = Oracle(g4, p4)
-            o4.set_fields(*f4)
-            c4 = time.perf_counter()
-            pr4 = o4.run_round_trips(2)
-            c5 = time.perf_counter()
-            s4 = co.Solver(g4, p4, device=0)
-            s4.set_fields(*f4)
-            pg4 = s4.runOPO(2)
-            e4 = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(s4.get_fields(), o4.get_fields())]
-            s4.close()
-            sweep_cpu[tag] = {"solve": k, "grid": [g4.nx, g4.ny, g4.nt], "trips": 2,
-                              "value": cells_of[k] * p4.n_slices * 2 / (c5 - c4), "unit": "cell-steps/s",
-                              "seconds": c5 - c4, "field_rel_l2_p_s_i": e4,
-                              "signal_power_rel_max": float(np.max(np.abs(pg4[:, 1] - pr4[:, 1]) / np.abs(pr4[:, 1])))}
-        out["cpu_baseline"]["sweep_grids"] = sweep_cpu
-    if not args.no_plain:
-        out["plain_rocfft"] = plain_leg()
-        if "value" in out["plain_rocfft"]:
-            out["plain_rocfft"]["fused_speedup"] = value / out["plain_rocfft"]["value"]
-    if args.horizon_trips > 0:
-        out["horizon"] = horizon_leg(co, configs, args.horizon_trips)
-    if args.sweep_trips > 0:
-        out["sweep"] = sweep_leg(co, configs, args.sweep_trips)
-    if args.serve_requests > 0:
-        out["serving"] = serving_leg(args.serve_requests, args.serve_rate)
-    print(json.dumps(out))
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        pg = s.runOPO(2)
+        errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(s.get_fields(), o.get_fields())]
+        s.close()
+        w = cells_of[k] * p.n_slices * 2
+        work += w
+        secs += c1 - c0
+        parts[tag] = {"solve": k, "grid": [g.nx, g.ny, g.nt], "trips": 2, "seconds": round(c1 - c0, 3),
+                      "value": w / (c1 - c0), "gpu_field_rel_l2_p_s_i": errs,
+                      "gpu_signal_power_rel_max": float(np.max(np.abs(pg[:, 1] - pr[:, 1]) / np.abs(pr[:, 1])))}
+    return {"value": work / secs, "unit": "cell-steps/s", "cores": get_threads(), "kind": "port",
+            "sample": f"FP64 oracle (own mixed-radix FFT; FFTW absent), smallest and largest grids of the "
+                      f"64-solve sweep, 2 round trips each ({secs:.1f} s)", "grids": parts}
+
+
+# --------------------------------------------------------------- extras
+def cavity_leg(co, configs, args, trips=20):
+    """The 64x64x256 singly-resonant cavity alone (FP32, tuned) and on the
+    all-FP64 device path, the long-horizon mode (its 200 trips hold 1e-5
+    field and 1e-6 power against the oracle, tests/test_gpu_horizon.py)."""
+    from cuopo3d_amd.capi import OPO_FP64
+    out = {}
+    for prec, name in ((0, "fp32"), (OPO_FP64, "fp64")):
+        g, p, _, f = configs.cavity_64(n_round_trips=trips + 1, precision=prec)
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        if not args.no_tune:
+            s.autotune(reps=args.tune_reps)
+        s.runOPO(1)
+        s.sync()
+        t0 = time.perf_counter()
+        s.runOPO(trips)
+        s.sync()
+        dt = time.perf_counter() - t0
+        s.close()
+        out[name] = {"value": g.nx * g.ny * g.nt * p.n_slices * trips / dt, "unit": "cell-steps/s",
+                     "ms_per_trip": dt / trips * 1e3, "trips": trips}
+    out["fp64_over_fp32"] = out["fp64"]["value"] / out["fp32"]["value"]
+    return out
+
+
+def single_pass_leg(co, configs):
+    """The single-pass workloads: the 32x32x64 plain single pass (100
+    slices) and the femtosecond 64x64x1024 pass (399 slices, also on the
+    FP64 path).  Their parity against the oracle is in the GPU tests."""
+    from cuopo3d_amd.capi import OPO_FP64
+    summary = {}
+    for name, make, reps, precs in (("single_pass_32x32x64", configs.single_pass_32, 20, (0,)),
+                                    ("femto_64x64x1024", configs.femto_64, 3, (0, OPO_FP64))):
+        for prec in precs:
+            g, p, _, f = make(precision=prec)
+            s = co.Solver(g, p, device=0)
+            s.set_fields(*f)
+            s.autotune(reps=3)
+            ts = []
+            for _ in range(reps + 1):   # the first pass warms up
+                s.set_fields(*f)
+                s.sync()
+                t0 = time.perf_counter()
+                s.runSinglePass()       # returns after the pass has finished on the GPU
+                ts.append(time.perf_counter() - t0)
+            dt = float(np.median(ts[1:]))
+            s.close()
+            leg = summary.setdefault(name, {"grid": [g.nx, g.ny, g.nt], "slices": p.n_slices})
+            leg["fp64" if prec else "fp32"] = {"value": g.nx * g.ny * g.nt * p.n_slices / dt,
+                                               "unit": "cell-steps/s", "ms_per_pass": dt * 1e3}
+    return summary
+
+
+def _batched_rate(co, solves, trips):
+    work = sum(g.nx * g.ny * g.nt * p.n_slices for (g, p, _, _) in solves)
+    b = co.Batch(200 << 30)
+    for (g, p, _, f) in solves:
+        b.submit(g, p, *f, trips + 2)
+    b.step()
+    b.step()
+    t0 = time.perf_counter()
+    for _ in range(trips):
+        b.step()
+    dt = time.perf_counter() - t0
+    b.close()
+    return work * trips / dt
+
+
+def _one_at_a_time_rate(co, solves, trips):
+    work, tot = 0.0, 0.0
+    for (g, p, _, f) in solves:
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        s.runOPO(1)
+        s.sync()
+        t0 = time.perf_counter()
+        s.runOPO(trips)
+        s.sync()
+        tot += time.perf_counter() - t0
+        work += g.nx * g.ny * g.nt * p.n_slices * trips
+        s.close()
+    return work / tot
+
+
+def small_sweep_leg(co, configs, n_solves=48, trips=5):
+    """A sweep of small grids (32x32x128 .. 64x64x256), where one solve
+    alone fills a fraction of the GPU: batched against one at a time."""
+    draws = configs.sweep_draws(n_solves=n_solves, seed=21)
+    sizes = [(32, 128), (32, 256), (64, 128), (64, 256)]
+    solves = []
+    for k, d in enumerate(draws):
+        d = dict(d)
+        d["nxy"], d["nt"] = sizes[k % len(sizes)]
+        solves.append(configs.sweep_solve(d, n_round_trips=100))
+    co.autotune_grids([(g.nx, g.ny, g.nt) for (g, _, _, _) in solves], reps=3)
+    batched = _batched_rate(co, solves, trips)
+    one = _one_at_a_time_rate(co, solves, trips)
+    return {"solves": n_solves, "grids": ["32x32x128", "32x32x256", "64x64x128", "64x64x256"],
+            "trips_timed": trips, "batched_cell_steps_per_s": batched,
+            "one_at_a_time_cell_steps_per_s": one, "speedup": batched / one}
+
+
+def one_at_a_time_leg(co, configs, batched, trips=3):
+    """The 64 sweep solves one after another through the single-solve API
+    (fused kernels, trip graphs), against the batched headline."""
+    solves = [configs.sweep_solve(d, n_round_trips=100) for d in configs.sweep_draws(n_solves=64, seed=3)]
+    one = _one_at_a_time_rate(co, solves, trips)
+    return {"trips_timed": trips, "cell_steps_per_s": one, "batched_speedup": batched / one}
+
+
+def reference_grid_leg(co, configs, tune_reps, trips=3):
+    """The cavity on the grid the reference compiles by default
+    (NX = NY = 128, NT = 2048: DtypesConsts.cuh:30-54), tuned for it."""
+    g, p, _, f = configs.cavity_64(n=(128, 128, 2048), n_round_trips=trips + 1)
+    s = co.Solver(g, p, device=0)
+    s.set_fields(*f)
+    report = s.autotune(reps=max(1, min(tune_reps, 2)))
+    s.runOPO(1)
+    s.sync()
+    t0 = time.perf_counter()
+    s.runOPO(trips)
+    s.sync()
+    dt = time.perf_counter() - t0
+    s.close()
+    cells = g.nx * g.ny * g.nt
+    return {"grid": [g.nx, g.ny, g.nt], "trips": trips, "value": cells * p.n_slices * trips / dt,
+            "unit": "cell-steps/s", "ms_per_step": dt / trips * 1e3,
+            "autotune": [ln[-12:] for ln in report.strip().splitlines()]}
+
+
+def plain_leg(one_at_a_time, batched, solves=8, trips=1):
+    """rocFFT + separate elementwise kernels (torch) one solve at a time on
+    the first sweep solves, in a child process (torch carries its own HIP
+    runtime); scaled to the whole set by the fused one-at-a-time rates."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-m", "cuopo3d_amd.baseline_rocfft", "--sweep-solves",
+                        str(solves), "--trips", str(trips)],
+                       cwd=ROOT, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip().splitlines()[-1] if r.stderr.strip() else "failed"}
+    pl = json.loads(r.stdout.strip().splitlines()[-1])
+    pl["sample"] = f"first {solves} solves of the sweep, {trips} trip each"
+    pl["batched_speedup_on_sample_rate"] = batched / pl["value"]
+    return pl
+
+
+def serving_leg(co, configs, overload):
+    """Continuous batching of Poisson arrivals over the serving mix
+    (32^2x128, 64^2x256, 96^2x512, 128^2x1024), 128 GB admission budget.
+    Under capacity: 1 arrival/s with 5-30 trips (about 40 % of the GPU),
+    so latency is service time; overloaded (optional): 10/s with 50-300
+    trips, where latency is queue length."""
+    from cuopo3d_amd import serving
+    co.autotune_grids([(nxy, nxy, nt) for nxy, nt in configs.SERVING_GRIDS], reps=3)
+    out = {}
+    cases = [("under_capacity", 1.0, (5, 30))]
+    if overload:
+        cases.append(("overloaded", 10.0, (50, 300)))
+    for name, rate, trips in cases:
+        reqs = serving.poisson_requests(16, rate_per_s=rate, seed=5, trips=trips)
+        rep = serving.serve(reqs, budget_bytes=128 << 30, keep_fields=False)
+        d = rep.as_dict()
+        d["arrival_rate_per_s"] = rate
+        d["trips"] = list(trips)
+        d["last_arrival_s"] = max(r.arrival for r in reqs)
+        d["errors"] = sum(1 for r in reqs if r.error)
+        out[name] = d
+    return out
 
 
 def horizon_leg(co, configs, trips):
@@ -262,215 +475,48 @@ def horizon_leg(co, configs, trips):
         po = np.concatenate(pw_o)[:, 1]
         res["fp32_signal_power_rel_max"].append(float(np.max(np.abs(np.concatenate(pw_32)[:, 1] - po) / po)))
         res["fp64_signal_power_rel_max"].append(float(np.max(np.abs(np.concatenate(pw_64)[:, 1] - po) / po)))
-        print(f"horizon trip {m}: fp32 {res['fp32_field_rel_l2_p_s_i'][-1]}", file=sys.stderr, flush=True)
+        log(f"horizon trip {m}: fp32 {res['fp32_field_rel_l2_p_s_i'][-1]}")
     s32.close()
     s64.close()
     return res
 
 
-def single_pass_leg(co, configs):
-    """The single-pass configs of the baseline plan on the GPU: the 32x32x64
-    plain single pass (100 slices) and the femtosecond 64x64x1024 single
-    pass (399 slices, also on the FP64 path).  Returns the JSON summary and
-    the output fields (for the oracle comparison of the CPU leg)."""
-    from cuopo3d_amd.capi import OPO_FP64
-    summary, fields = {}, {}
-    for name, make, reps, precs in (("single_pass_32x32x64", configs.single_pass_32, 20, (0,)),
-                                    ("femto_64x64x1024", configs.femto_64, 3, (0, OPO_FP64))):
-        for prec in precs:
-            g, p, _, f = make(precision=prec)
-            s = co.Solver(g, p, device=0)
-            s.set_fields(*f)
-            for line in s.autotune(reps=3).strip().splitlines():
-                print(f"{name} autotune:", line.strip(), file=sys.stderr)
-            ts = []
-            for _ in range(reps + 1):   # the first pass warms up
-                s.set_fields(*f)
-                s.sync()
-                t0 = time.perf_counter()
-                s.runSinglePass()       # returns after the pass has finished on the GPU
-                ts.append(time.perf_counter() - t0)
-            dt = float(np.median(ts[1:]))
-            fields[(name, prec)] = s.get_fields()
-            s.close()
-            leg = summary.setdefault(name, {"grid": [g.nx, g.ny, g.nt], "slices": p.n_slices})
-            leg["fp64" if prec else "fp32"] = {"value": g.nx * g.ny * g.nt * p.n_slices / dt,
-                                               "unit": "cell-steps/s", "ms_per_pass": dt * 1e3}
-    return summary, fields
-
-
-def reference_grid_leg(co, configs, tune_reps, trips=3):
-    """The same cavity on the grid the reference compiles by default
-    (NX = NY = 128, NT = 2048: DtypesConsts.cuh:30-54), tuned for it."""
-    g, p, _, f = configs.cavity_64(n=(128, 128, 2048), n_round_trips=trips + 1)
-    s = co.Solver(g, p, device=0)
-    s.set_fields(*f)
-    report = s.autotune(reps=max(1, min(tune_reps, 2)))
-    s.runOPO(1)
-    s.sync()
-    t0 = time.perf_counter()
-    s.runOPO(trips)
-    s.sync()
-    dt = time.perf_counter() - t0
-    s.close()
-    cells = g.nx * g.ny * g.nt
-    return {"grid": [g.nx, g.ny, g.nt], "trips": trips, "value": cells * p.n_slices * trips / dt,
-            "unit": "cell-steps/s", "ms_per_step": dt / trips * 1e3,
-            "autotune": [ln[-12:] for ln in report.strip().splitlines()]}
-
-
-def fp64_leg(co, configs, trips):
-    """The same cavity on the all-FP64 device path (the long-horizon mode)."""
-    from cuopo3d_amd.capi import OPO_FP64
-    g, p, _, f = configs.cavity_64(n_round_trips=trips + 1, precision=OPO_FP64)
-    s = co.Solver(g, p, device=0)
-    s.set_fields(*f)
-    s.autotune(reps=3)
-    s.runOPO(1)
-    s.sync()
-    t0 = time.perf_counter()
-    s.runOPO(trips)
-    s.sync()
-    dt = time.perf_counter() - t0
-    s.close()
-    cells = g.nx * g.ny * g.nt
-    return {"trips": trips, "value": cells * p.n_slices * trips / dt, "unit": "cell-steps/s",
-            "ms_per_step": dt / trips * 1e3}
-
-
-def idler_dumped_leg(co, configs, trips):
-    """The same cavity with the non-resonant idler removed at the mirror
-    (multiplier 0) instead of the reference's multiplier 1."""
-    g, p, _, f = configs.cavity_64(n_round_trips=trips + 1)
-    p.fb_re[1], p.fb_im[1] = 0.0, 0.0
-    s = co.Solver(g, p, device=0)
-    s.set_fields(*f)
-    s.runOPO(1)
-    s.sync()
-    t0 = time.perf_counter()
-    pw = s.runOPO(trips)
-    s.sync()
-    dt = time.perf_counter() - t0
-    s.close()
-    return {"trips": trips, "ms_per_step": dt / trips * 1e3,
-            "signal_power_last_trip_W": float(pw[-1, 1]), "idler_power_last_trip_W": float(pw[-1, 2])}
-
-
-def _counter_summary(grid, key, stage):
-    """Figure `key` of a stage's kernel from the committed counter summary
-    of this grid, or None when no summary covers it."""
-    import re
-    path = os.path.join(ROOT, "profiles", f"counters_cavity_{grid.nx}x{grid.ny}x{grid.nt}.md")
-    try:
-        text = open(path).read()
-    except OSError:
-        return None
-    m = re.search(key + r":.*\b" + re.escape(stage) + r"=(\d+)", text)
-    return int(m.group(1)) if m else None
-
-
-def measured_traffic(grid, stage):
-    """HBM bytes per launch of a stage's kernel from the committed counter
-    summary (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
-    128 MiB device copy), or None when no summary covers this grid."""
-    return _counter_summary(grid, "traffic_bytes_per_launch", stage)
-
-
-def valu_utilisation(grid, stage, avg_s):
-    """The kernels are bound by FP32 vector arithmetic, not by HBM: VALU
-    lane operations per cell (rocprofv3 SQ_INSTS_VALU x 64 / cells, from the
-    committed counter summary) times the cells of a launch, over its mean
-    duration, against 256 CUs x 64 lanes x 2.4 GHz (FP32 without packed
-    math, one lane operation per lane per clock; an FMA counts once)."""
-    ops = _counter_summary(grid, "valu_lane_ops_per_cell", stage)
-    if ops is None or avg_s <= 0:
-        return None
+# ------------------------------------------------ single-cavity profiling
+def cavity_profile_mode(co, configs, lib, args):
+    """One cavity solve on the given grid, trips replayed as a graph, and
+    per-stage kernel times with HIP events around each launch of a second
+    run without graph capture (opo_profile_trips)."""
+    from cuopo3d_amd.capi import STAGES
+    n = (args.cavity_grid[0], args.cavity_grid[0], args.cavity_grid[1])
+    grid, p, ph, fields = configs.cavity_64(n=n, n_round_trips=args.warmup + args.steps + 1)
     cells = grid.nx * grid.ny * grid.nt
-    achieved = ops * cells / avg_s
-    return {"kernel": stage, "lane_ops_per_cell": ops, "achieved": round(achieved / 1e12, 2),
-            "peak": VALU_PEAK_T, "unit": "T lane-ops/s", "frac": round(achieved / 1e12 / VALU_PEAK_T, 4)}
-
-
-def plain_leg(trips=4):
-    """rocFFT + separate elementwise kernels (torch) on the same cavity workload,
-    in a child process (torch carries its own HIP runtime)."""
-    import subprocess
-    r = subprocess.run([sys.executable, "-m", "cuopo3d_amd.baseline_rocfft", "--trips", str(trips)],
-                       cwd=ROOT, capture_output=True, text=True, timeout=600)
-    if r.returncode != 0:
-        return {"error": r.stderr.strip().splitlines()[-1] if r.stderr.strip() else "failed"}
-    return json.loads(r.stdout.strip().splitlines()[-1])
-
-
-def sweep_leg(co, configs, trips, n_solves=64, plain_solves=8, plain_trips=1):
-    """64 uneven cavity solves batched (one launch per stage per size class, trip as a
-    graph) against the same 64 solves run one at a time through the single-solve
-    API, and against the plain rocFFT path one solve at a time (first solves)."""
-    draws = configs.sweep_draws(n_solves=n_solves, seed=3)
-    solves = [configs.sweep_solve(d, n_round_trips=trips) for d in draws]
-    for line in co.autotune_grids([(g.nx, g.ny, g.nt) for (g, _, _, _) in solves], reps=3):
-        print("sweep autotune:", line.strip().replace("\n", " | "), file=sys.stderr)
-    work = sum(g.nx * g.ny * g.nt * p.n_slices for (g, p, _, _) in solves)
-    b = co.Batch(200 << 30)
-    tickets = [b.submit(g, p, *f, trips) for (g, p, _, f) in solves]
-    b.step()  # admission, graph capture, first trip (not timed)
+    s = co.Solver(grid, p, device=0)
+    s.set_fields(*fields)
+    if args.variants_in:
+        variants_set(lib, json.load(open(args.variants_in)))
+    elif not args.no_tune:
+        s.autotune(reps=args.tune_reps)
+    if args.variants_out:
+        json.dump(variants_get(lib), open(args.variants_out, "w"))
+    s.runOPO(args.warmup)
+    s.sync()
     t0 = time.perf_counter()
-    for _ in range(trips - 1):
-        b.step()
+    s.runOPO(args.steps)
+    s.sync()
     dt = time.perf_counter() - t0
-    batched = work * (trips - 1) / dt
-    for t in tickets:
-        b.poll(t, want_fields=False)
-    b.close()
-    # the same 64 solves one at a time, each warmed up by one trip
-    one_trips = 2
-    tot = 0.0
-    sub_rates = []
-    for k, (g, p, _, f) in enumerate(solves):
-        s = co.Solver(g, p, device=0)
-        s.set_fields(*f)
-        s.runOPO(1)
-        t0 = time.perf_counter()
-        s.runOPO(one_trips)
-        s.sync()
-        e = time.perf_counter() - t0
-        tot += e
-        if k < plain_solves:
-            sub_rates.append((g.nx * g.ny * g.nt * p.n_slices * one_trips, e))
-        s.close()
-    one = work * one_trips / tot
-    out = {"solves": n_solves, "trips_timed": trips - 1, "batched_cell_steps_per_s": batched,
-           "one_at_a_time_cell_steps_per_s": one, "speedup_vs_fused_one_at_a_time": batched / one}
-    # plain rocFFT path, one solve at a time, in a child process
-    import subprocess
-    r = subprocess.run([sys.executable, "-m", "cuopo3d_amd.baseline_rocfft", "--sweep-solves",
-                        str(plain_solves), "--trips", str(plain_trips)],
-                       cwd=ROOT, capture_output=True, text=True, timeout=900)
-    if r.returncode == 0:
-        pl = json.loads(r.stdout.strip().splitlines()[-1])
-        fused_sub = sum(w for w, _ in sub_rates) / sum(e for _, e in sub_rates)
-        out["plain_rocfft_one_at_a_time_cell_steps_per_s"] = pl["value"]
-        out["plain_rocfft_sample"] = f"first {plain_solves} solves of the sweep, {plain_trips} trip each"
-        # the plain rate on the sample, scaled to the whole set by the fused one-at-a-time
-        # rates of the same sample and of the whole set (the sizes differ)
-        out["plain_rocfft_whole_set_estimate"] = pl["value"] * one / fused_sub
-        out["speedup_vs_plain_one_at_a_time"] = batched / out["plain_rocfft_whole_set_estimate"]
-    else:
-        out["plain_rocfft_error"] = r.stderr.strip().splitlines()[-1] if r.stderr.strip() else "failed"
-    return out
-
-
-def serving_leg(n, rate):
-    import cuopo3d_amd as co
-    from cuopo3d_amd import serving, configs
-    for line in co.autotune_grids([(nxy, nxy, nt) for nxy, nt in configs.SERVING_GRIDS], reps=3):
-        print("serving autotune:", line.strip().replace("\n", " | "), file=sys.stderr)
-    reqs = serving.poisson_requests(n, rate_per_s=rate, seed=5)
-    rep = serving.serve(reqs, budget_bytes=128 << 30, keep_fields=False)
-    d = rep.as_dict()
-    d["arrival_rate_per_s"] = rate
-    d["errors"] = sum(1 for r in reqs if r.error)
-    return d
+    ms = (C.c_double * 6)()
+    nl = (C.c_int * 6)()
+    co.check(lib.opo_profile_trips(s.h, 5, ms, nl), "opo_profile_trips")
+    stages = {}
+    for k, name in enumerate(STAGES):
+        if nl[k]:
+            avg = ms[k] / nl[k]
+            alg = cells * STAGE_FIELDS[name] * 8 * STAGE_RW.get(name, 2)
+            stages[name] = {"launches": nl[k], "avg_us": avg * 1e3, "alg_bytes": alg,
+                            "GBs": alg / (avg * 1e-3) / 1e9}
+    return {"workload": f"cavity {grid.nx}x{grid.ny}x{grid.nt}", "value": cells * p.n_slices * args.steps / dt,
+            "unit": "cell-steps/s", "ms_per_trip": dt / args.steps * 1e3, "stages_unbatched_events": stages,
+            "variants": variants_get(lib)}
 
 
 if __name__ == "__main__":

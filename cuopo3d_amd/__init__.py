@@ -228,5 +228,19 @@ class Batch:
               "opo_batch_stats")
         return r.value, q.value, u.value
 
+    def profile(self, on=True):
+        """Time every launch inside the replayed trip graph from the next step on."""
+        check(self.lib.opo_batch_profile(self.h, 1 if on else 0), "opo_batch_profile")
+
+    def profile_read(self):
+        """Per (stage, size class): launches, summed ms, algorithmic bytes per launch."""
+        from .capi import StageTime, STAGES
+        n = C.c_int()
+        check(self.lib.opo_batch_profile_read(self.h, None, 0, C.byref(n)), "opo_batch_profile_read")
+        buf = (StageTime * max(n.value, 1))()
+        check(self.lib.opo_batch_profile_read(self.h, buf, n.value, C.byref(n)), "opo_batch_profile_read")
+        return [{"stage": STAGES[e.stage], "size": e.size, "launches": e.launches, "ms": e.ms,
+                 "alg_bytes": e.alg_bytes} for e in buf[:n.value]]
+
     def footprint(self, grid, n_trips=1):
         return self.lib.opo_batch_footprint(C.byref(grid), self.precision, int(n_trips))

@@ -105,6 +105,12 @@ class Crystal(C.Structure):
     ]
 
 
+class StageTime(C.Structure):
+    """opo_stage_time (include/opo3d.h)."""
+    _fields_ = [("stage", C.c_int), ("size", C.c_int), ("launches", C.c_int),
+                ("ms", C.c_double), ("alg_bytes", C.c_double)]
+
+
 _lib = None
 
 
@@ -156,6 +162,8 @@ def load():
         "opo_batch_poll": (ip, [vp, C.c_uint64, C.POINTER(ip), dp, vp, vp, vp]),
         "opo_batch_footprint": (sz, [C.POINTER(Grid), ip, ip]),
         "opo_batch_stats": (ip, [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(sz)]),
+        "opo_batch_profile": (ip, [vp, ip]),
+        "opo_batch_profile_read": (ip, [vp, C.POINTER(StageTime), ip, C.POINTER(ip)]),
         # host set-up (include/opo3d_setup.h)
         "opo_slice_count": (ip, [C.c_double, ip]),
         "opo_crystal_properties": (ip, [ip, C.c_double, C.c_double, C.c_double, C.c_double,

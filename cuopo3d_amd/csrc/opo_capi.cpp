@@ -973,10 +973,61 @@ struct opo_batch {
     std::vector<TripState> h_ts;
     std::vector<int> h_status;
 
+    // Kernel timing inside the replayed trip graph (opo_batch_profile):
+    // events recorded around every launch at capture, read after each step.
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_entry;    // entry index of the pair (2k, 2k + 1)
+    size_t ev_used = 0;
+    std::vector<opo_stage_time> prof;
+    int prof_entry(int stage, int size, double alg) {
+        for (size_t k = 0; k < prof.size(); ++k)
+            if (prof[k].stage == stage && prof[k].size == size) {
+                prof[k].alg_bytes = alg;
+                return (int)k;
+            }
+        prof.push_back(opo_stage_time{stage, size, 0, 0.0, alg});
+        return (int)prof.size() - 1;
+    }
+    int mark(int stage, int size, double alg) {
+        if (!profiling) return OPO_OK;
+        if (ev_used + 1 >= ev_pool.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                CK(hipEventCreate(&e));
+                ev_pool.push_back(e);
+            }
+        }
+        if (ev_used % 2 == 0) ev_entry.push_back(prof_entry(stage, size, alg));
+        CK(hipEventRecord(ev_pool[ev_used++], stream));
+        return OPO_OK;
+    }
+    // After a step: add the elapsed time of every bracketed launch.
+    int collect() {
+        if (!profiling) return OPO_OK;
+        for (size_t k = 0; k + 1 < ev_used; k += 2) {
+            float t = 0.f;
+            CK(hipEventElapsedTime(&t, ev_pool[k], ev_pool[k + 1]));
+            opo_stage_time &e = prof[ev_entry[k / 2]];
+            e.ms += t;
+            e.launches += 1;
+        }
+        return OPO_OK;
+    }
+    // Algorithmic bytes of one launch over the resident solves selected.
+    template <typename Pred> double class_bytes(int fields, int rw, Pred pick) const {
+        const double cb = prec == OPO_FP64 ? 16.0 : 8.0;
+        double cells = 0.0;
+        for (const BatchEntry *e : resident)
+            if (pick(e)) cells += (double)e->g.nx * e->g.ny * e->g.nt;
+        return cells * fields * cb * rw;
+    }
+
     ~opo_batch() {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         if (graph) (void)hipGraphExecDestroy(graph);
+        for (auto e : ev_pool) (void)hipEventDestroy(e);
         for (auto &kv : all)
             if (kv.second->d_series) (void)hipFree(kv.second->d_series);
         all.clear();
@@ -1082,6 +1133,8 @@ struct opo_batch {
         h_ts.assign(n, TripState{0, 0});
         h_status.assign(n, 0);
         // capture one round trip of every resident solve
+        ev_used = 0;
+        ev_entry.clear();
         hipGraph_t gr = nullptr;
         CK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         int rc = fp64 ? enqueue_trip<double>() : enqueue_trip<float>();
@@ -1101,23 +1154,53 @@ struct opo_batch {
         return OPO_OK;
     }
 
+#define BMARK(stage, size, alg)                    \
+    do {                                           \
+        int mrc_ = mark(stage, size, alg);         \
+        if (mrc_) return mrc_;                     \
+    } while (0)
     template <typename T> int enqueue_trip() {
         const SolveDesc<T> *sd = static_cast<const SolveDesc<T> *>(d_desc);
+        std::vector<double> tb, xb, fb;
+        for (const Range &r : trange) tb.push_back(class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nt == r.cls; }));
+        for (const Range &r : xyrange) xb.push_back(class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nx == r.cls; }));
+        for (const Range &r : fsrange)
+            fb.push_back(class_bytes(2, 2, [&](const BatchEntry *e) { return e->g.nx == r.cls && e->p.fs_on; }));
+        const double pb = class_bytes(3, 1, [](const BatchEntry *) { return true; });
         for (int k = 0; k < n_slices; ++k) {
-            for (const Range &r : trange)
+            for (size_t c = 0; c < trange.size(); ++c) {
+                const Range &r = trange[c];
+                BMARK(OPO_STAGE_TIME_OP, r.cls, tb[c]);
                 CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off,
                                         k == 0 ? BATCH_FIRST : BATCH_MID, r.v));
-            for (const Range &r : xyrange)
+                BMARK(OPO_STAGE_TIME_OP, r.cls, tb[c]);
+            }
+            for (size_t c = 0; c < xyrange.size(); ++c) {
+                const Range &r = xyrange[c];
+                BMARK(OPO_STAGE_XY_OP, r.cls, xb[c]);
                 CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0, r.v));
+                BMARK(OPO_STAGE_XY_OP, r.cls, xb[c]);
+            }
         }
-        for (const Range &r : trange)
+        for (size_t c = 0; c < trange.size(); ++c) {
+            const Range &r = trange[c];
+            BMARK(OPO_STAGE_GDD, r.cls, tb[c]);
             CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST, r.v));
-        for (const Range &r : fsrange)
+            BMARK(OPO_STAGE_GDD, r.cls, tb[c]);
+        }
+        for (size_t c = 0; c < fsrange.size(); ++c) {
+            const Range &r = fsrange[c];
+            BMARK(OPO_STAGE_FREE_SPACE, r.cls, fb[c]);
             CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 1, r.v));
+            BMARK(OPO_STAGE_FREE_SPACE, r.cls, fb[c]);
+        }
+        BMARK(OPO_STAGE_POWER, 0, pb);
         CK(launch_batch_power<T>(stream, prange.n, (int)resident.size(), sd, d_ts, d_status,
                                  d_work + prange.off, d_partial, max_chunk));
+        BMARK(OPO_STAGE_POWER, 0, pb);
         return OPO_OK;
     }
+#undef BMARK
 };
 
 extern "C" {
@@ -1350,6 +1433,7 @@ int opo_batch_step(opo_batch *b, int *n_active) {
     CK(hipGraphLaunch(b->graph, b->stream));
     CK(hipMemcpyAsync(b->h_status.data(), b->d_status, sizeof(int) * n, hipMemcpyDeviceToHost, b->stream));
     CK(hipStreamSynchronize(b->stream));
+    if ((rc = b->collect())) return rc;
     std::vector<BatchEntry *> keep;
     for (int i = 0; i < n; ++i) {
         BatchEntry *e = b->resident[i];
@@ -1394,6 +1478,30 @@ int opo_batch_poll(opo_batch *b, uint64_t ticket, int *done, double *power, void
     }
     b->all.erase(it);
     if (rc) return fail(rc, "%s", msg.c_str());
+    return OPO_OK;
+}
+
+int opo_batch_profile(opo_batch *b, int on) {
+    if (!b) return fail(OPO_ERR_ARG, "null batch");
+    std::lock_guard<std::mutex> lk(b->mu);
+    const bool want = on != 0;
+    if (want != b->profiling) {   // re-capture with or without the events
+        b->profiling = want;
+        b->dirty = true;
+        b->prof.clear();
+    }
+    for (opo_stage_time &e : b->prof) {   // start the sums again
+        e.launches = 0;
+        e.ms = 0.0;
+    }
+    return OPO_OK;
+}
+
+int opo_batch_profile_read(opo_batch *b, opo_stage_time *out, int cap, int *n) {
+    if (!b || !n || (cap > 0 && !out)) return fail(OPO_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    *n = (int)b->prof.size();
+    for (int k = 0; k < cap && k < *n; ++k) out[k] = b->prof[k];
     return OPO_OK;
 }
 

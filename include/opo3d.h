@@ -265,6 +265,28 @@ size_t opo_batch_footprint(const opo_grid *grid, int precision, int n_trips);
 int opo_batch_stats(opo_batch *b, int *resident, int *queued,
                     size_t *bytes_used);
 
+/* Kernel timing of batched round trips (no counterpart in the reference,
+ * which only prints wall time, Common.cuh:83-90).  With profiling on, the
+ * trip graph is captured with a HIP event recorded before and after every
+ * launch, so each launch is timed inside the replayed graph, and every
+ * opo_batch_step adds the elapsed times to one entry per (stage, size
+ * class).  Turning it on or off re-captures the graph at the next step
+ * and drops the entries; every call sets the sums back to zero. */
+typedef struct {
+    int stage;          /* OPO_STAGE_TIME_OP (first and middle slices),
+                           OPO_STAGE_GDD (trip-end pass), OPO_STAGE_XY_OP,
+                           OPO_STAGE_FREE_SPACE or OPO_STAGE_POWER          */
+    int size;           /* NT of a t-pass class, NX of an xy class, 0 power */
+    int launches;       /* launches timed                                   */
+    double ms;          /* their summed duration                            */
+    double alg_bytes;   /* algorithmic bytes of one launch: 16 B (FP64 32 B)
+                           per cell per field read and written (the power
+                           sum reads only), over the class's solves         */
+} opo_stage_time;
+int opo_batch_profile(opo_batch *b, int on);
+/* Copies up to cap entries to out; *n receives the number of entries. */
+int opo_batch_profile_read(opo_batch *b, opo_stage_time *out, int cap, int *n);
+
 #ifdef __cplusplus
 }
 #endif
