@@ -245,7 +245,7 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
         return fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
     }
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
-    if (n < 16) return v == 0;
+    if (n < 16 || (fp64 && n == 128)) return v == 0;   // FP64 128: the two-pass form only
     if (n <= 32) return v < 6;
     if (xy_variant_persistent(n, v)) return !fp64 && n * n / xy_variant_bs(n, v) <= 16;
     return true;
@@ -381,6 +381,60 @@ __global__ void __launch_bounds__(BS) k_xyplanep(FieldOps<T> ops, int nt, int nf
     xy_run<T, NX, BS>(XySolo<T, NX>{ops, nt, nf, tw}, lds, stab);
 }
 
+// FP64 planes of 128 x 128 (256 KiB) do not fit LDS, so their xy operator
+// runs as two passes through HBM: Q_x on bands of RB consecutive rows, then
+// Q_y on bands of CB consecutive columns (rows of the band are CB * 16 B
+// contiguous in HBM).  Each line goes through exactly the operations of the
+// one-plane kernel.
+template <typename T, int NX, int RB, int BS>
+__global__ void __launch_bounds__(BS) k_xrows(FieldOps<T> ops, int ny, const cplx<T> *__restrict__ tw,
+                                              const int *status) {
+    if (status && *status) return;
+    constexpr int NXP = NX + 1;
+    __shared__ cplx<T> lds[RB * NXP];
+    const int bands = ny / RB;
+    const int t = blockIdx.x / bands, r0 = (blockIdx.x % bands) * RB, f = blockIdx.y;
+    cplx<T> *A = ops.f[f] + ((size_t)t * ny + r0) * NX;
+    for (int i = threadIdx.x; i < RB * NX; i += BS) lds[(i / NX) * NXP + (i % NX)] = A[i];
+    __syncthreads();
+    line_op<T, NX, -1, NXP, 1>(lds, RB, tw, ops.tab[f], threadIdx.x, BS);
+    for (int i = threadIdx.x; i < RB * NX; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
+}
+
+template <typename T, int NY, int CB, int BS>
+__global__ void __launch_bounds__(BS) k_ycols(FieldOps<T> ops, int nx, const cplx<T> *__restrict__ tw,
+                                              const int *status) {
+    if (status && *status) return;
+    constexpr int CBP = CB + 1;
+    __shared__ cplx<T> lds[NY * CBP];
+    const int bands = nx / CB;
+    const int t = blockIdx.x / bands, c0 = (blockIdx.x % bands) * CB, f = blockIdx.y;
+    cplx<T> *A = ops.f[f] + (size_t)t * NY * nx + c0;
+    for (int i = threadIdx.x; i < NY * CB; i += BS) lds[(i / CB) * CBP + (i % CB)] = A[(size_t)(i / CB) * nx + (i % CB)];
+    __syncthreads();
+    // lines are the CB columns (line c starts at c, its elements CBP apart)
+    line_op<T, NY, -1, 1, CBP>(lds, CB, tw, ops.taby[f], threadIdx.x, BS);
+    for (int i = threadIdx.x; i < NY * CB; i += BS) A[(size_t)(i / CB) * nx + (i % CB)] = lds[(i / CB) * CBP + (i % CB)];
+}
+
+// Bands of 32 rows or columns: 256 line tasks per phase of a 128-point
+// line operator (16 x 8 split) for 256 threads, 66 KiB of LDS (two
+// workgroups per CU).
+constexpr int kSplitBand = 32;
+
+template <typename T, int NX>
+static hipError_t xy_split_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
+                                  const cplx<T> *tw, const int *status) {
+    if (g.ny != NX || NX % kSplitBand != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_xrows<T, NX, kSplitBand, 256>), dim3((unsigned)(g.nt * (NX / kSplitBand)), (unsigned)nf),
+                       dim3(256), 0, st, ops, g.ny, tw, status);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_ycols<T, NX, kSplitBand, 256>), dim3((unsigned)(g.nt * (NX / kSplitBand)), (unsigned)nf),
+                       dim3(256), 0, st, ops, g.nx, tw, status);
+    return hipGetLastError();
+}
+
 template <typename T, int NX, int BS, int P>
 static hipError_t xy_launch_groups(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                                    const cplx<T> *tw, const int *status) {
@@ -427,8 +481,7 @@ static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<
 bool xyplane_supported(int nx, int ny, bool fp64) {
     if (nx != ny) return false;
     switch (nx) {
-        case 1: case 16: case 32: case 64: case 96: return true;
-        case 128: return !fp64;
+        case 1: case 16: case 32: case 64: case 96: case 128: return true;
         default: return false;
     }
 }
@@ -445,7 +498,7 @@ hipError_t launch_xyplane(hipStream_t st, const Geometry &g, const FieldOps<T> &
         case 96: return xy_dispatch<T, 96>(st, g, ops, nf, tw, status);
         case 128:
             if constexpr (sizeof(T) == 4) return xy_dispatch<T, 128>(st, g, ops, nf, tw, status);
-            else return hipErrorInvalidValue;
+            else return xy_split_launch<T, 128>(st, g, ops, nf, tw, status);
         default: return hipErrorInvalidValue;
     }
 }
