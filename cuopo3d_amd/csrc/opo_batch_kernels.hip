@@ -10,6 +10,7 @@
 #include "opo_kernels.hpp"
 #include "opo_passes.hpp"
 #include "opo_rk4.hpp"
+#include "opo_power.hpp"
 
 namespace opo {
 
@@ -163,13 +164,9 @@ __global__ void __launch_bounds__(BS) kb_xy(const SolveDesc<T> *__restrict__ sd,
     xy_plane<T, NX, BS, P>(A, tx, ty, d.twx, lds);
 }
 
-__device__ __forceinline__ void neumaier(double &sum, double &comp, double v) {
-    double t = sum + v;
-    if (fabs(sum) >= fabs(v)) comp += (sum - t) + v;
-    else comp += (v - t) + sum;
-    sum = t;
-}
-
+// Power of every resident solve, in the order a solve uses alone
+// (opo_power.hpp): work item (solve, b) is partial block b of the solve's
+// power_blocks(cells) blocks, then one 256-thread workgroup per solve.
 template <typename T>
 __global__ void __launch_bounds__(256) kb_power_partial(const SolveDesc<T> *__restrict__ sd,
                                                         const int *__restrict__ status,
@@ -180,46 +177,27 @@ __global__ void __launch_bounds__(256) kb_power_partial(const SolveDesc<T> *__re
     if (status[w.solve]) return;
     const SolveDesc<T> &d = sd[w.solve];
     const size_t n = (size_t)d.nx * d.ny * d.nt;
-    const size_t per = (n + d.nchunk - 1) / d.nchunk;
-    const size_t lo = per * w.item, hi = lo + per < n ? lo + per : n;
-    const int tid = threadIdx.x;
-    for (int f = 0; f < 3; ++f) {
-        const cplx<T> *A = f == 0 ? d.f[0] : (f == 1 ? d.f[1] : d.f[2]);
-        double s = 0.0, c = 0.0;
-        for (size_t i = lo + tid; i < hi; i += 256) {
-            const double re = (double)A[i].x, im = (double)A[i].y;
-            neumaier(s, c, fma(re, re, im * im));
-        }
-        red[f][tid] = s + c;
-    }
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if (tid < k)
-            for (int f = 0; f < 3; ++f) red[f][tid] += red[f][tid + k];
-        __syncthreads();
-    }
-    if (tid == 0)
-        for (int f = 0; f < 3; ++f)
-            partial[((size_t)w.solve * 3 + f) * max_chunk + w.item] = red[f][0];
+    power_partial_block<T>(d.f[0], d.f[1], d.f[2], n, w.item, d.nchunk, red,
+                           partial + (size_t)w.solve * 3 * max_chunk + w.item, max_chunk);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(64) kb_power_final(const SolveDesc<T> *__restrict__ sd,
-                                                     const TripState *__restrict__ ts, int *status,
-                                                     const double *partial, int max_chunk) {
+__global__ void __launch_bounds__(256) kb_power_final(const SolveDesc<T> *__restrict__ sd,
+                                                      const TripState *__restrict__ ts, int *status,
+                                                      const double *partial, int max_chunk) {
+    __shared__ double red[3][256];
     const int s = blockIdx.x;
-    if (threadIdx.x != 0 || status[s]) return;
+    if (status[s]) return;
     const SolveDesc<T> &d = sd[s];
-    double pw[3];
-    for (int f = 0; f < 3; ++f) {
-        double sum = 0.0, comp = 0.0;
-        for (int c = 0; c < d.nchunk; ++c) neumaier(sum, comp, partial[((size_t)s * 3 + f) * max_chunk + c]);
-        pw[f] = d.pref[f] * (sum + comp);
+    power_final_sum(partial + (size_t)s * 3 * max_chunk, max_chunk, d.nchunk, red);
+    if (threadIdx.x == 0) {
+        double pw[3];
+        for (int f = 0; f < 3; ++f) pw[f] = d.pref[f] * red[f][0];
+        const int k = ts[s].trip;
+        if (k < d.series_cap)
+            for (int f = 0; f < 3; ++f) d.series[3 * (size_t)k + f] = pw[f];
+        if (!isfinite(pw[1])) status[s] = k + 1;
     }
-    const int k = ts[s].trip;
-    if (k < d.series_cap)
-        for (int f = 0; f < 3; ++f) d.series[3 * (size_t)k + f] = pw[f];
-    if (!isfinite(pw[1])) status[s] = k + 1;
 }
 
 bool batch_supported(int nx, int ny, int nt, bool fp64) {
@@ -328,7 +306,7 @@ hipError_t launch_batch_power(hipStream_t st, int n_work, int n_solves, const So
                        partial, max_chunk);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((kb_power_final<T>), dim3((unsigned)n_solves), dim3(64), 0, st, sd, ts, status,
+    hipLaunchKernelGGL((kb_power_final<T>), dim3((unsigned)n_solves), dim3(256), 0, st, sd, ts, status,
                        partial, max_chunk);
     return hipGetLastError();
 }

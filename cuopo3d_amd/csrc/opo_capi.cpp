@@ -34,6 +34,7 @@
 #include "opo_device.hpp"
 #include "opo_internal.hpp"
 #include "opo_kernels.hpp"
+#include "opo_power.hpp"
 
 namespace opo {
 
@@ -326,12 +327,6 @@ template <typename T> struct SolveT final : SolveImpl {
             qfsx[f] = tables + o.fx + 2 * (size_t)f * g.nx;
             qfsy[f] = tables + o.fy + 2 * (size_t)f * g.ny;
         }
-    }
-
-    // Power partials: enough blocks to spread the sum, fixed per grid.
-    static int power_blocks(size_t cells) {
-        const size_t want = (cells + 256 * 16 - 1) / (256 * 16);
-        return (int)(want < 1 ? 1 : (want > 1024 ? 1024 : want));
     }
 
     int build_tables() {
@@ -1039,11 +1034,9 @@ struct opo_batch {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    static int chunks_for(const opo_grid &g) {
-        size_t n = (size_t)g.nx * g.ny * g.nt;
-        size_t c = n / 65536;
-        return (int)(c < 1 ? 1 : (c > 64 ? 64 : c));
-    }
+    // Power partial blocks of a solve: as many as the solve uses alone, so
+    // its per-trip powers are bitwise the same in the batch.
+    static int chunks_for(const opo_grid &g) { return power_blocks((size_t)g.nx * g.ny * g.nt); }
 
     int rebuild() {
         if (graph) {

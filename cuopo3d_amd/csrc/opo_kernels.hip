@@ -19,6 +19,7 @@
 #include "opo_kernels.hpp"
 #include "opo_rk4.hpp"
 #include "opo_passes.hpp"
+#include "opo_power.hpp"
 
 namespace opo {
 
@@ -504,41 +505,13 @@ hipError_t launch_xyplane(hipStream_t st, const Geometry &g, const FieldOps<T> &
 }
 
 // --------------------------------------------------------------- power
-__device__ __forceinline__ void neumaier_add(double &sum, double &comp, double v) {
-    double t = sum + v;
-    if (fabs(sum) >= fabs(v)) comp += (sum - t) + v;
-    else comp += (v - t) + sum;
-    sum = t;
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256) k_power_partial(const cplx<T> *f0, const cplx<T> *f1,
                                                        const cplx<T> *f2, size_t n, double *partial,
                                                        const int *status) {
     if (status && *status) return;
     __shared__ double red[3][256];
-    const cplx<T> *F[3] = {f0, f1, f2};
-    const int tid = threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (int f = 0; f < 3; ++f) {
-        double s = 0.0, c = 0.0;
-        if (F[f]) {
-            for (size_t i = (size_t)blockIdx.x * blockDim.x + tid; i < n; i += stride) {
-                cplx<T> a = F[f][i];
-                double re = (double)a.x, im = (double)a.y;
-                neumaier_add(s, c, fma(re, re, im * im));
-            }
-        }
-        red[f][tid] = s + c;
-    }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w)
-            for (int f = 0; f < 3; ++f) red[f][tid] += red[f][tid + w];
-        __syncthreads();
-    }
-    if (tid == 0)
-        for (int f = 0; f < 3; ++f) partial[(size_t)f * gridDim.x + blockIdx.x] = red[f][0];
+    power_partial_block<T>(f0, f1, f2, n, blockIdx.x, gridDim.x, red, partial + blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(256) k_power_final(const double *partial, int nblk, double pref0,
@@ -547,19 +520,8 @@ __global__ void __launch_bounds__(256) k_power_final(const double *partial, int 
                                                      int series_cap) {
     if (status && *status) return;
     __shared__ double red[3][256];
-    const int tid = threadIdx.x;
-    for (int f = 0; f < 3; ++f) {
-        double s = 0.0, c = 0.0;
-        for (int b = tid; b < nblk; b += 256) neumaier_add(s, c, partial[(size_t)f * nblk + b]);
-        red[f][tid] = s + c;
-    }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w)
-            for (int f = 0; f < 3; ++f) red[f][tid] += red[f][tid + w];
-        __syncthreads();
-    }
-    if (tid == 0) {
+    power_final_sum(partial, nblk, nblk, red);
+    if (threadIdx.x == 0) {
         double pw[3] = {pref0 * red[0][0], pref1 * red[1][0], pref2 * red[2][0]};
         int k = 0;
         if (trip) {
