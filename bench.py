@@ -94,20 +94,30 @@ def main():
         print(json.dumps(cavity_profile_mode(co, configs, lib, args)))
         return
 
-    out = sweep_headline(co, configs, lib, args)
+    t_start = time.perf_counter()
+
+    def leg(name, fn, *a):
+        t0 = time.perf_counter()
+        log(f"bench: {name} ...")
+        r = fn(*a)
+        log(f"bench: {name} done in {time.perf_counter() - t0:.1f} s ({time.perf_counter() - t_start:.0f} s)")
+        return r
+
+    out = leg("batched sweep (headline)", sweep_headline, co, configs, lib, args)
+    log(f"bench: headline {out['value']:.4g} cell-steps/s, roofline {out['roofline']}")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_leg(co, configs)
+        out["cpu_baseline"] = leg("CPU baseline (FP64 oracle)", cpu_leg, co, configs)
     if not args.headline_only:
-        out["cavity_64x64x256"] = cavity_leg(co, configs, args)
-        out["single_pass"] = single_pass_leg(co, configs)
-        out["small_grid_sweep"] = small_sweep_leg(co, configs)
-        out["one_at_a_time"] = one_at_a_time_leg(co, configs, out["value"])
-        out["reference_default_grid"] = reference_grid_leg(co, configs, args.tune_reps)
+        out["cavity_64x64x256"] = leg("cavity 64x64x256", cavity_leg, co, configs, args)
+        out["single_pass"] = leg("single passes", single_pass_leg, co, configs)
+        out["small_grid_sweep"] = leg("small-grid sweep", small_sweep_leg, co, configs)
+        out["one_at_a_time"] = leg("sweep one at a time", one_at_a_time_leg, co, configs, out["value"])
+        out["reference_default_grid"] = leg("128x128x2048", reference_grid_leg, co, configs, args.tune_reps)
         if not args.no_plain:
-            out["plain_rocfft"] = plain_leg(out["one_at_a_time"], out["value"])
-        out["serving"] = serving_leg(co, configs, args.serve_overload)
+            out["plain_rocfft"] = leg("plain rocFFT", plain_leg, out["one_at_a_time"], out["value"])
+        out["serving"] = leg("serving", serving_leg, co, configs, args.serve_overload)
         if args.horizon_trips > 0:
-            out["horizon"] = horizon_leg(co, configs, args.horizon_trips)
+            out["horizon"] = leg("horizon", horizon_leg, co, configs, args.horizon_trips)
     print(json.dumps(out))
 
 
@@ -134,6 +144,7 @@ def sweep_headline(co, configs, lib, args, n_solves=64, trips=100):
     solves = [configs.sweep_solve(d, n_round_trips=trips) for d in draws]
     grids = sorted({(g.nx, g.ny, g.nt) for (g, _, _, _) in solves})
     tune = []
+    log(f"sweep: {n_solves} solves set up")
     if args.variants_in:
         variants_set(lib, json.load(open(args.variants_in)))
         tune = ["variants set from " + os.path.basename(args.variants_in)]

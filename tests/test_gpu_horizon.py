@@ -22,11 +22,11 @@ from helpers import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-CHECKPOINTS = (1, 10, 50, 100, 200)
+CHECKPOINTS = (1, 10, 25, 50, 75, 100, 125, 150, 175, 200)
 
 
 @pytest.mark.timeout(1500)
-def test_cavity_200_trips_against_oracle(require_gpu):
+def test_cavity_200_trips_against_oracle(require_gpu, capfd):
     g, p64, ph, f64 = configs.cavity_64(n_round_trips=200, precision=OPO_FP64)
     _, p32, _, _ = configs.cavity_64(n_round_trips=200, precision=OPO_FP32)
     f32 = [x.astype(np.complex64) for x in f64]
@@ -57,7 +57,10 @@ def test_cavity_200_trips_against_oracle(require_gpu):
                 worst["fp64"] = max(worst["fp64"], max(errs))
             elif cp <= 10:
                 assert max(errs) < 1e-5 and perr < 1e-6, (cp, errs, perr)
-        print(" | ".join(line), flush=True)
+        # the oracle needs about a second per trip on 16 cores: each
+        # checkpoint is reported as it is reached, past the output capture
+        with capfd.disabled():
+            print(" | ".join(line), flush=True)
     s64.close()
     s32.close()
     assert worst["fp64"] < 1e-9   # six orders of magnitude of margin (measured ~1e-11)
