@@ -24,6 +24,7 @@ OPO_FP32 = 0
 OPO_FP64 = 1
 OPO_TUNE_TIME_PASS = 0
 OPO_TUNE_XY_PASS = 1
+OPO_NOALLOC_SCRATCH_BYTES = 1 << 20   # include/opo3d.h
 
 STAGES = ("nonlinear", "time_op", "xy_op", "free_space", "gdd", "power")
 
@@ -145,6 +146,8 @@ def load():
         "opo_ssfm": (ip, [vp]),
         "opo_single_pass": (ip, [vp]),
         "opo_run_round_trips": (ip, [vp, ip, dp]),
+        "opo_run_round_trips_noalloc": (ip, [C.POINTER(Grid), C.POINTER(Params), ip, vp, vp, vp, vp,
+                                             ip, dp]),
         "opo_trips_done": (ip, [vp]),
         "opo_power": (ip, [vp, dp]),
         "opo_power_time": (ip, [vp, C.c_int, C.c_int, C.c_int, C.c_double, dp]),

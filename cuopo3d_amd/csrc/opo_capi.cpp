@@ -154,11 +154,14 @@ template <typename T> struct SolveT final : SolveImpl {
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     int graph_variants[2] = {-1, -1};   // (t, xy) launch variants the graphs were captured with
 
+    bool owns = true;   // false: every device buffer belongs to someone else
+
     ~SolveT() override {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &gx : graph)
             if (gx) (void)hipGraphExecDestroy(gx);
+        if (!owns) return;
         (void)hipFree(Api);
         for (auto *a : A) (void)hipFree(a);
         (void)hipFree(tables);
@@ -327,6 +330,30 @@ template <typename T> struct SolveT final : SolveImpl {
             qfsx[f] = tables + o.fx + 2 * (size_t)f * g.nx;
             qfsy[f] = tables + o.fy + 2 * (size_t)f * g.ny;
         }
+    }
+
+    // Run on buffers owned by the caller (the no-allocation entry point):
+    // fields api, a0..a2, a table block already filled, dbuf_ of
+    // 3 power_blocks(N) + 4 doubles, flags_ of 4 ints and a power series of
+    // series_cap_ trips.  Launches go to the null stream, without graphs.
+    void init_borrowed(C *api, C *a0, C *a1, C *a2, C *block, double *dbuf_, int *flags_, double *series_,
+                       int series_cap_) {
+        owns = false;
+        P = (size_t)g.nx * g.ny;
+        N = P * g.nt;
+        Api = api;
+        A[0] = a0;
+        A[1] = a1;
+        A[2] = a2;
+        bind_tables(block);
+        nblk = power_blocks(N);
+        dbuf = dbuf_;
+        partial = dbuf;
+        pw3 = dbuf + 3 * (size_t)nblk;
+        flags = flags_;
+        series = series_;
+        series_cap = series_cap_;
+        use_graphs = false;
     }
 
     int build_tables() {
@@ -581,6 +608,7 @@ template <typename T> struct SolveT final : SolveImpl {
 
     int ensure_series(int n) {
         if (n <= series_cap) return OPO_OK;
+        if (!owns) return fail(OPO_ERR_ARG, "%d trips exceed the %d-trip power series provided", n, series_cap);
         for (auto &gx : graph)
             if (gx) { (void)hipGraphExecDestroy(gx); gx = nullptr; }
         CK(hipStreamSynchronize(stream));
@@ -891,6 +919,78 @@ static int make_solve(const opo_grid *g, const opo_params *p, int device,
     rc = s->init();
     if (rc) return rc;
     out = std::move(s);
+    return OPO_OK;
+}
+
+// ------------------------------------------------ no-allocation scratch
+// One fixed block of device memory per device for the entry point that
+// allocates nothing (opo_run_round_trips_noalloc): the table block, the
+// power partials and counters, and the per-trip power series of the call.
+// Each device holds its own copy of the array at its own address, found
+// once per device with hipGetSymbolAddress; host code never uses the
+// array's name as an address.
+static __device__ __align__(256) unsigned char scratch[OPO_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_noalloc_mu;              // one no-allocation call at a time per process
+static void *g_scratch_ptr[64];              // device address of scratch per device
+alignas(64) static unsigned char g_stage[OPO_NOALLOC_SCRATCH_BYTES];  // host image of the tables
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Bytes of scratch a call needs: tables, partials + pw3, flags, series.
+static size_t noalloc_bytes(const opo_grid &g, int prec, int n) {
+    const size_t cb = prec == OPO_FP64 ? 16 : 8;
+    const size_t N = (size_t)g.nx * g.ny * g.nt;
+    const int nblk = power_blocks(N);
+    return align256(SolveT<float>::table_entries(g) * cb) + align256(sizeof(double) * (3 * (size_t)nblk + 4)) +
+           align256(4 * sizeof(int)) + align256(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+}
+
+template <typename T>
+static int noalloc_run(const opo_grid &g, const opo_params &p, int device, unsigned char *base, const void *api,
+                       void *ap, void *as, void *ai, int n, double *power) {
+    using C = cplx<T>;
+    SolveT<T> s;
+    s.g = g;
+    s.p = p;
+    s.p.omega = nullptr;
+    s.prec = p.precision;
+    s.device = device;
+    // host image: omega (NT doubles) and the time grid first, then the tables
+    double *w = reinterpret_cast<double *>(g_stage);
+    double *tg = w + g.nt;
+    C *h = reinterpret_cast<C *>(g_stage + align256(2 * sizeof(double) * (size_t)g.nt));
+    const size_t ne = SolveT<T>::table_entries(g);
+    if ((unsigned char *)(h + ne) > g_stage + sizeof(g_stage))
+        return fail(OPO_ERR_UNSUPPORTED, "grid %dx%dx%d: tables exceed the %zu-byte staging area", g.nx, g.ny,
+                    g.nt, sizeof(g_stage));
+    if (p.omega) for (int i = 0; i < g.nt; ++i) w[i] = p.omega[i];
+    else omega_rule(g.nt, p.t_window, w);
+    SolveT<T>::fill_tables(g, s.p, w, h, tg);
+    // device layout inside scratch: tables | partials + pw3 | flags | series
+    const size_t N = (size_t)g.nx * g.ny * g.nt;
+    const int nblk = power_blocks(N);
+    unsigned char *q = base;
+    C *block = reinterpret_cast<C *>(q);
+    q += align256(ne * sizeof(C));
+    double *dbuf = reinterpret_cast<double *>(q);
+    q += align256(sizeof(double) * (3 * (size_t)nblk + 4));
+    int *flags = reinterpret_cast<int *>(q);
+    q += align256(4 * sizeof(int));
+    double *series = reinterpret_cast<double *>(q);
+    s.init_borrowed((C *)api, (C *)ap, (C *)as, (C *)ai, block, dbuf, flags, series, n);
+    CK(hipMemcpy(block, h, ne * sizeof(C), hipMemcpyHostToDevice));
+    int rc = s.enqueue_trips(n);   // clears the status word and trip counter first
+    if (rc) {
+        (void)hipStreamSynchronize(nullptr);
+        return rc;
+    }
+    int hflags[2] = {0, 0};
+    CK(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, nullptr));
+    CK(hipStreamSynchronize(nullptr));
+    const int done = hflags[0] ? hflags[0] : n;
+    if (power) CK(hipMemcpy(power, series, sizeof(double) * 3 * (size_t)done, hipMemcpyDeviceToHost));
+    if (hflags[0]) return fail(OPO_ERR_NAN, "non-finite signal power at round trip %d", done - 1);
     return OPO_OK;
 }
 
@@ -1255,6 +1355,35 @@ int opo_run_round_trips(opo_solve *s, int n, double *power) {
     if (rc) return rc;
     return s->s->finish_trips(n, power);
 }
+int opo_run_round_trips_noalloc(const opo_grid *grid, const opo_params *params, int device, const void *api,
+                                void *ap, void *as, void *ai, int n, double *power) {
+    int rc = validate(grid, params);
+    if (rc) return rc;
+    if (!api || !ap || !as || !ai) return fail(OPO_ERR_ARG, "null field pointer");
+    if (n < 1) return fail(OPO_ERR_ARG, "n must be >= 1");
+    if (params->n_slices < 1) return fail(OPO_ERR_ARG, "round trips need n_slices >= 1");
+    const size_t need = noalloc_bytes(*grid, params->precision, n);
+    if (need > OPO_NOALLOC_SCRATCH_BYTES)
+        return fail(OPO_ERR_UNSUPPORTED,
+                    "grid %dx%dx%d with %d trips needs %zu bytes of scratch; the no-allocation entry point has %d",
+                    grid->nx, grid->ny, grid->nt, n, need, (int)OPO_NOALLOC_SCRATCH_BYTES);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(OPO_ERR_NOGPU, "no HIP device");
+    if (device < 0 && hipGetDevice(&device) != hipSuccess) device = 0;
+    if (device >= ndev || device >= 64) return fail(OPO_ERR_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    CK(hipSetDevice(device));
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        CK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(OPO_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[device] = ptr;
+    }
+    unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[device]);
+    if (params->precision == OPO_FP64) return noalloc_run<double>(*grid, *params, device, base, api, ap, as, ai, n, power);
+    return noalloc_run<float>(*grid, *params, device, base, api, ap, as, ai, n, power);
+}
+
 int opo_trips_done(const opo_solve *s) { return (s && s->s) ? s->s->trips_done : 0; }
 int opo_power(opo_solve *s, double out[3]) {
     SOLVE_OR_FAIL(s);

@@ -181,6 +181,30 @@ int opo_single_pass(opo_solve *s);
 int opo_run_round_trips(opo_solve *s, int n, double *power);
 int opo_trips_done(const opo_solve *s);
 
+/* opo_run_round_trips without allocating device memory.  No counterpart in
+ * the reference: it is for callers that cannot call hipMalloc (code
+ * between HIP graph captures, tight serving loops, harnesses that forbid
+ * allocation).  Runs round trips 0..n-1 of the solve described by grid and
+ * params (the free-space step is skipped on trip params->n_round_trips - 1
+ * as in opo_run_round_trips) on fields the caller keeps in device memory of
+ * `device` (< 0: the current device): api, the injected pump (read), and
+ * ap, as, ai (read and written), [t][y][x] interleaved complex of the
+ * solve's precision.  Same kernels, same launch variants and the same
+ * fields as opo_create + opo_set_fields + opo_run_round_trips, bitwise.
+ * power (nullable, host) receives n*3 doubles as there.  The tables, power
+ * partials, counters and power series live in one fixed per-device array
+ * of OPO_NOALLOC_SCRATCH_BYTES, which the device holds until the process
+ * ends; a grid and trip count whose scratch exceeds it is rejected with
+ * OPO_ERR_UNSUPPORTED before the GPU is touched (FP32 128x128x2048 leaves
+ * room for about 33,000 trips, FP64 for about 24,000).  Calls in one
+ * process are serialised; calls that overlap on one GPU from different
+ * processes each have their own array.  Synchronous: launches go one by
+ * one (no graph) to the device's null stream, and the call returns when
+ * they have finished. */
+#define OPO_NOALLOC_SCRATCH_BYTES (1 << 20)
+int opo_run_round_trips_noalloc(const opo_grid *grid, const opo_params *params,
+                                int device, const void *api, void *ap,
+                                void *as, void *ai, int n, double *power);
 /* averagePowerCutXY(A) for p, s, i (Solver3D.cuh:529-545), compensated. */
 int opo_power(opo_solve *s, double out[3]);
 /* averagePowerTime(A, x, y) (Solver3D.cuh:509-523): *out = prefactor *

@@ -96,3 +96,22 @@ def test_tune_set_rejects_missing_variants():
     assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, 256, keep) == capi.OPO_OK
     assert lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, 256) == keep
 
+
+def test_noalloc_rejects_oversized_calls_before_touching_the_gpu():
+    """The no-allocation entry point checks its fixed scratch (tables,
+    partials, counters, 3 doubles of power per trip) before any HIP call."""
+    lib = capi.load()
+    assert capi.OPO_NOALLOC_SCRATCH_BYTES == 1 << 20
+    g, p, _, _ = configs.cavity_64(n=(128, 128, 2048), n_round_trips=1)
+    dummy = C.c_void_p(16)   # never dereferenced: the size check comes first
+    rc = lib.opo_run_round_trips_noalloc(C.byref(g), C.byref(p), 0, dummy, dummy, dummy, dummy,
+                                         40000, None)
+    assert rc == OPO_ERR_UNSUPPORTED
+    assert b"scratch" in lib.opo_last_error()
+    assert lib.opo_run_round_trips_noalloc(C.byref(g), C.byref(p), 0, None, dummy, dummy, dummy,
+                                           1, None) == OPO_ERR_ARG
+    assert lib.opo_run_round_trips_noalloc(C.byref(g), C.byref(p), 0, dummy, dummy, dummy, dummy,
+                                           0, None) == OPO_ERR_ARG
+    if lib.opo_device_count() == 0:
+        assert lib.opo_run_round_trips_noalloc(C.byref(g), C.byref(p), 0, dummy, dummy, dummy,
+                                               dummy, 10, None) == OPO_ERR_NOGPU
