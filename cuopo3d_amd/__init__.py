@@ -228,6 +228,13 @@ class Batch:
               "opo_batch_stats")
         return r.value, q.value, u.value
 
+    def rebuilds(self):
+        """(re-captures of the trip graph, of which updated in place, host seconds)."""
+        n, u, sec = C.c_int(), C.c_int(), C.c_double()
+        check(self.lib.opo_batch_rebuilds(self.h, C.byref(n), C.byref(u), C.byref(sec)),
+              "opo_batch_rebuilds")
+        return n.value, u.value, sec.value
+
     def profile(self, on=True):
         """Time every launch inside the replayed trip graph from the next step on."""
         check(self.lib.opo_batch_profile(self.h, 1 if on else 0), "opo_batch_profile")
@@ -241,6 +248,16 @@ class Batch:
         check(self.lib.opo_batch_profile_read(self.h, buf, n.value, C.byref(n)), "opo_batch_profile_read")
         return [{"stage": STAGES[e.stage], "size": e.size, "launches": e.launches, "ms": e.ms,
                  "alg_bytes": e.alg_bytes} for e in buf[:n.value]]
+
+    def time_stages(self, reps=3):
+        """Each (stage, size class) timed alone as a replayed graph of its
+        launches of one trip; advances the resident fields."""
+        from .capi import StageTime, STAGES
+        buf = (StageTime * 64)()
+        n = C.c_int()
+        check(self.lib.opo_batch_time_stages(self.h, int(reps), buf, 64, C.byref(n)), "opo_batch_time_stages")
+        return [{"stage": STAGES[e.stage], "size": e.size, "launches": e.launches, "ms": e.ms,
+                 "alg_bytes": e.alg_bytes} for e in buf[:min(n.value, 64)]]
 
     def footprint(self, grid, n_trips=1):
         return self.lib.opo_batch_footprint(C.byref(grid), self.precision, int(n_trips))

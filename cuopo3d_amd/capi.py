@@ -166,7 +166,9 @@ def load():
         "opo_batch_footprint": (sz, [C.POINTER(Grid), ip, ip]),
         "opo_batch_stats": (ip, [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(sz)]),
         "opo_batch_profile": (ip, [vp, ip]),
+        "opo_batch_rebuilds": (ip, [vp, C.POINTER(ip), C.POINTER(ip), dp]),
         "opo_batch_profile_read": (ip, [vp, C.POINTER(StageTime), ip, C.POINTER(ip)]),
+        "opo_batch_time_stages": (ip, [vp, ip, C.POINTER(StageTime), ip, C.POINTER(ip)]),
         # host set-up (include/opo3d_setup.h)
         "opo_slice_count": (ip, [C.c_double, ip]),
         "opo_crystal_properties": (ip, [ip, C.c_double, C.c_double, C.c_double, C.c_double,

@@ -59,7 +59,7 @@ __device__ __forceinline__ TLine3Args<T> batch_time_args(const SolveDesc<T> &d, 
     return a;
 }
 
-template <typename T, int NT, int TX, int BS>
+template <typename T, int NT, int TX, int BS, bool FUSED = false>
 __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ sd,
                                               const int *__restrict__ status,
                                               const WorkItem *__restrict__ work, int mode) {
@@ -68,7 +68,10 @@ __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ s
     const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
     if (status[w.solve]) return;
     const SolveDesc<T> &d = sd[w.solve];
-    nl_time_tile<T, NT, TX, BS, false>(batch_time_args<T, NT>(d, mode), d.nx, d.ny, w.item, d.twt, lds, twl);
+    if constexpr (FUSED)
+        nl_time_tile_fused<T, NT, TX, BS, false>(batch_time_args<T, NT>(d, mode), d.nx, d.ny, w.item, d.twt, lds, twl);
+    else
+        nl_time_tile<T, NT, TX, BS, false>(batch_time_args<T, NT>(d, mode), d.nx, d.ny, w.item, d.twt, lds, twl);
 }
 
 // Work units of the fused pass over a work list: tile w.item of solve w.solve.
@@ -222,8 +225,8 @@ static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
                     hipLaunchKernelGGL((kb_timep<T, NT, Vr::tx, Vr::bs>), dim3(persistent_grid(n_work, Vr::per_cu)),
                                        dim3(Vr::bs), 0, st, sd, status, work, n_work, mode);
                 else
-                    hipLaunchKernelGGL((kb_time<T, NT, Vr::tx, Vr::bs>), dim3((unsigned)n_work), dim3(Vr::bs), 0,
-                                       st, sd, status, work, mode);
+                    hipLaunchKernelGGL((kb_time<T, NT, Vr::tx, Vr::bs, Vr::fused>), dim3((unsigned)n_work),
+                                       dim3(Vr::bs), 0, st, sd, status, work, mode);
                 e = hipGetLastError();
             }
         }

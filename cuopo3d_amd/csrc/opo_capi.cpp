@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <chrono>
 #include <deque>
 #include <map>
 #include <memory>
@@ -799,6 +800,7 @@ template <typename T> struct SolveT final : SolveImpl {
                 const int pc = tune_per_cu(fp64, axis, n, v);
                 char pers[32] = "";
                 if (pc) std::snprintf(pers, sizeof(pers), ", persistent %d/CU", pc);
+                else if (axis == 0 && tune_fused(fp64, n, v)) std::snprintf(pers, sizeof(pers), ", fused");
                 else if (axis == 1 && xy_group(n, v) > 1)
                     std::snprintf(pers, sizeof(pers), ", %d planes", xy_group(n, v));
                 std::snprintf(buf, sizeof(buf), " v%d(tile %d, block %d%s) %.2f;", v,
@@ -1094,18 +1096,28 @@ struct opo_batch {
             }
         }
         if (ev_used % 2 == 0) ev_entry.push_back(prof_entry(stage, size, alg));
-        CK(hipEventRecord(ev_pool[ev_used++], stream));
+        // recorded as an event node of the captured trip graph
+        CK(hipEventRecordWithFlags(ev_pool[ev_used++], stream, hipEventRecordExternal));
         return OPO_OK;
     }
     // After a step: add the elapsed time of every bracketed launch.
+    // A runtime that cannot time event nodes of a graph leaves the sums at
+    // zero and the reason in prof_error; the step itself is unaffected.
+    std::string prof_error;
     int collect() {
-        if (!profiling) return OPO_OK;
+        if (!profiling || !prof_error.empty()) return OPO_OK;
         for (size_t k = 0; k + 1 < ev_used; k += 2) {
             float t = 0.f;
-            CK(hipEventElapsedTime(&t, ev_pool[k], ev_pool[k + 1]));
-            opo_stage_time &e = prof[ev_entry[k / 2]];
-            e.ms += t;
-            e.launches += 1;
+            const hipError_t e = hipEventElapsedTime(&t, ev_pool[k], ev_pool[k + 1]);
+            if (e != hipSuccess) {
+                prof_error = std::string("timing the events of the trip graph: ") + hipGetErrorString(e);
+                (void)hipGetLastError();
+                for (opo_stage_time &x : prof) x.launches = 0, x.ms = 0.0;
+                return OPO_OK;
+            }
+            opo_stage_time &x = prof[ev_entry[k / 2]];
+            x.ms += t;
+            x.launches += 1;
         }
         return OPO_OK;
     }
@@ -1138,21 +1150,33 @@ struct opo_batch {
     // its per-trip powers are bitwise the same in the batch.
     static int chunks_for(const opo_grid &g) { return power_blocks((size_t)g.nx * g.ny * g.nt); }
 
+    // Device buffers of the resident set, grown (never shrunk) as it needs:
+    // an admission or a finished solve reuses them without freeing.
+    size_t cap_desc = 0, cap_status = 0, cap_ts = 0, cap_work = 0, cap_partial = 0;
+    template <typename U> int ensure_buf(U *&ptr, size_t &cap, size_t bytes) {
+        if (bytes <= cap) return OPO_OK;
+        (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        const size_t want = std::max(bytes, (size_t)4096);
+        if (hipMalloc((void **)&ptr, want) != hipSuccess)
+            return fail(OPO_ERR_NOMEM, "batch descriptor allocation of %zu bytes failed", want);
+        cap = want;
+        return OPO_OK;
+    }
+    // Rebuild bookkeeping: count, graphs updated in place, host seconds.
+    int n_rebuilds = 0, n_updated = 0;
+    double rebuild_s = 0.0;
+
     int rebuild() {
-        if (graph) {
-            (void)hipGraphExecDestroy(graph);
-            graph = nullptr;
-        }
-        (void)hipFree(d_desc);
-        (void)hipFree(d_status);
-        (void)hipFree(d_ts);
-        (void)hipFree(d_work);
-        (void)hipFree(d_partial);
-        d_desc = nullptr;
-        d_status = nullptr;
-        d_ts = nullptr;
-        d_work = nullptr;
-        d_partial = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = rebuild_impl();
+        rebuild_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        n_rebuilds += 1;
+        return rc;
+    }
+
+    int rebuild_impl() {
         const int n = (int)resident.size();
         const size_t dsz = prec == OPO_FP64 ? sizeof(SolveDesc<double>) : sizeof(SolveDesc<float>);
         std::vector<unsigned char> hdesc(dsz * n);
@@ -1214,12 +1238,15 @@ struct opo_batch {
         for (int i = 0; i < n; ++i)
             for (int c = 0; c < chunks_for(resident[i]->g); ++c) work.push_back(WorkItem{i, c});
         prange.n = (int)work.size() - prange.off;
-        if (hipMalloc(&d_desc, std::max<size_t>(hdesc.size(), 1)) != hipSuccess ||
-            hipMalloc(&d_status, sizeof(int) * std::max(n, 1)) != hipSuccess ||
-            hipMalloc(&d_ts, sizeof(TripState) * std::max(n, 1)) != hipSuccess ||
-            hipMalloc(&d_work, sizeof(WorkItem) * std::max<size_t>(work.size(), 1)) != hipSuccess ||
-            hipMalloc(&d_partial, sizeof(double) * 3 * (size_t)std::max(n, 1) * max_chunk) != hipSuccess)
-            return fail(OPO_ERR_NOMEM, "batch descriptor allocation failed");
+        // Grow the buffers where needed.  A buffer that moves changes the
+        // captured launch arguments, which the update below carries over.
+        int rc0;
+        if ((rc0 = ensure_buf(d_desc, cap_desc, std::max<size_t>(hdesc.size(), 1))) ||
+            (rc0 = ensure_buf(d_status, cap_status, sizeof(int) * std::max(n, 1))) ||
+            (rc0 = ensure_buf(d_ts, cap_ts, sizeof(TripState) * std::max(n, 1))) ||
+            (rc0 = ensure_buf(d_work, cap_work, sizeof(WorkItem) * std::max<size_t>(work.size(), 1))) ||
+            (rc0 = ensure_buf(d_partial, cap_partial, sizeof(double) * 3 * (size_t)std::max(n, 1) * max_chunk)))
+            return rc0;
         CK(hipMemcpyAsync(d_desc, hdesc.data(), hdesc.size(), hipMemcpyHostToDevice, stream));
         CK(hipMemcpyAsync(d_work, work.data(), sizeof(WorkItem) * work.size(), hipMemcpyHostToDevice, stream));
         CK(hipMemsetAsync(d_status, 0, sizeof(int) * std::max(n, 1), stream));
@@ -1237,9 +1264,29 @@ struct opo_batch {
             return rc;
         }
         if (e != hipSuccess) return fail(OPO_ERR_HIP, "batch capture: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
+        // Update the instantiated graph in place when the new trip has the
+        // same shape (same launches in the same order: the size classes did
+        // not change, only their work lists); instantiate it otherwise.
+        if (graph) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            const hipError_t eu = hipGraphExecUpdate(graph, gr, &err_node, &res);
+            if (eu == hipSuccess && res == hipGraphExecUpdateSuccess) {
+                n_updated += 1;
+            } else {
+                (void)hipGetLastError();   // a refused update is not an error of the batch
+                (void)hipGraphExecDestroy(graph);
+                graph = nullptr;
+            }
+        }
+        if (!graph) {
+            e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(gr);
+                return fail(OPO_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
+            }
+        }
         (void)hipGraphDestroy(gr);
-        if (e != hipSuccess) return fail(OPO_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
         built_variants.clear();
         for (const Range &r : trange) built_variants.push_back(r.v);
         for (const Range &r : xyrange) built_variants.push_back(r.v);
@@ -1247,53 +1294,134 @@ struct opo_batch {
         return OPO_OK;
     }
 
-#define BMARK(stage, size, alg)                    \
-    do {                                           \
-        int mrc_ = mark(stage, size, alg);         \
-        if (mrc_) return mrc_;                     \
-    } while (0)
+    // Launch filter of enqueue_trip: with only_stage >= 0, only the launches
+    // of that (stage, size class) are enqueued (the stage timing graphs).
+    int only_stage = -1, only_size = -1, n_enqueued = 0;
+    bool wanted(int stage, int size) const {
+        return only_stage < 0 || (stage == only_stage && size == only_size);
+    }
+
+    // Enqueue one round trip of every resident solve: per slice the fused
+    // t pass of each NT class then the xy pass of each NX class; then the
+    // trip-end pass, free space and the power sums.  Each launch goes through
+    // launch(): the filter above, and the profiling events around it.
     template <typename T> int enqueue_trip() {
         const SolveDesc<T> *sd = static_cast<const SolveDesc<T> *>(d_desc);
+        n_enqueued = 0;
+        auto launch = [&](int stage, int size, double alg, auto &&fn) -> int {
+            if (!wanted(stage, size)) return OPO_OK;
+            int rc = mark(stage, size, alg);
+            if (rc) return rc;
+            CK(fn());
+            n_enqueued += 1;
+            return mark(stage, size, alg);
+        };
         std::vector<double> tb, xb, fb;
         for (const Range &r : trange) tb.push_back(class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nt == r.cls; }));
         for (const Range &r : xyrange) xb.push_back(class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nx == r.cls; }));
         for (const Range &r : fsrange)
             fb.push_back(class_bytes(2, 2, [&](const BatchEntry *e) { return e->g.nx == r.cls && e->p.fs_on; }));
         const double pb = class_bytes(3, 1, [](const BatchEntry *) { return true; });
+        int rc;
         for (int k = 0; k < n_slices; ++k) {
             for (size_t c = 0; c < trange.size(); ++c) {
                 const Range &r = trange[c];
-                BMARK(OPO_STAGE_TIME_OP, r.cls, tb[c]);
-                CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off,
-                                        k == 0 ? BATCH_FIRST : BATCH_MID, r.v));
-                BMARK(OPO_STAGE_TIME_OP, r.cls, tb[c]);
+                if ((rc = launch(OPO_STAGE_TIME_OP, r.cls, tb[c], [&] {
+                         return launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off,
+                                                     k == 0 ? BATCH_FIRST : BATCH_MID, r.v);
+                     })))
+                    return rc;
             }
             for (size_t c = 0; c < xyrange.size(); ++c) {
                 const Range &r = xyrange[c];
-                BMARK(OPO_STAGE_XY_OP, r.cls, xb[c]);
-                CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0, r.v));
-                BMARK(OPO_STAGE_XY_OP, r.cls, xb[c]);
+                if ((rc = launch(OPO_STAGE_XY_OP, r.cls, xb[c], [&] {
+                         return launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0, r.v);
+                     })))
+                    return rc;
             }
         }
         for (size_t c = 0; c < trange.size(); ++c) {
             const Range &r = trange[c];
-            BMARK(OPO_STAGE_GDD, r.cls, tb[c]);
-            CK(launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST, r.v));
-            BMARK(OPO_STAGE_GDD, r.cls, tb[c]);
+            if ((rc = launch(OPO_STAGE_GDD, r.cls, tb[c], [&] {
+                     return launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST, r.v);
+                 })))
+                return rc;
         }
         for (size_t c = 0; c < fsrange.size(); ++c) {
             const Range &r = fsrange[c];
-            BMARK(OPO_STAGE_FREE_SPACE, r.cls, fb[c]);
-            CK(launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 1, r.v));
-            BMARK(OPO_STAGE_FREE_SPACE, r.cls, fb[c]);
+            if ((rc = launch(OPO_STAGE_FREE_SPACE, r.cls, fb[c], [&] {
+                     return launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 1, r.v);
+                 })))
+                return rc;
         }
-        BMARK(OPO_STAGE_POWER, 0, pb);
-        CK(launch_batch_power<T>(stream, prange.n, (int)resident.size(), sd, d_ts, d_status,
-                                 d_work + prange.off, d_partial, max_chunk));
-        BMARK(OPO_STAGE_POWER, 0, pb);
-        return OPO_OK;
+        return launch(OPO_STAGE_POWER, 0, pb, [&] {
+            return launch_batch_power<T>(stream, prange.n, (int)resident.size(), sd, d_ts, d_status,
+                                         d_work + prange.off, d_partial, max_chunk);
+        });
     }
-#undef BMARK
+
+    // Time each (stage, size class) of the trip by itself: a graph holding
+    // that class's launches of one round trip, in trip order and with the
+    // trip's arguments, replayed reps times between two HIP events on the
+    // batch stream.  The solves' fields go through these launches too.
+    int time_stages(int reps, std::vector<opo_stage_time> &out) {
+        out.clear();
+        const bool fp64 = prec == OPO_FP64;
+        // the (stage, size) entries of the trip, in trip order
+        std::vector<std::pair<int, int>> keys;
+        for (const Range &r : trange) keys.push_back({OPO_STAGE_TIME_OP, r.cls});
+        for (const Range &r : xyrange) keys.push_back({OPO_STAGE_XY_OP, r.cls});
+        for (const Range &r : trange) keys.push_back({OPO_STAGE_GDD, r.cls});
+        for (const Range &r : fsrange) keys.push_back({OPO_STAGE_FREE_SPACE, r.cls});
+        keys.push_back({OPO_STAGE_POWER, 0});
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        CK(hipEventCreate(&ev[0]));
+        CK(hipEventCreate(&ev[1]));
+        const bool keep_prof = profiling;
+        profiling = false;
+        int rc = OPO_OK;
+        for (const auto &key : keys) {
+            only_stage = key.first;
+            only_size = key.second;
+            hipGraph_t gr = nullptr;
+            hipGraphExec_t ge = nullptr;
+            if (hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+                rc = fail(OPO_ERR_HIP, "stage timing: capture failed");
+                break;
+            }
+            rc = fp64 ? enqueue_trip<double>() : enqueue_trip<float>();
+            const int launches = n_enqueued;
+            const hipError_t e = hipStreamEndCapture(stream, &gr);
+            if (!rc && e != hipSuccess) rc = fail(OPO_ERR_HIP, "stage timing capture: %s", hipGetErrorString(e));
+            if (!rc && hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess)
+                rc = fail(OPO_ERR_HIP, "stage timing: instantiate failed");
+            if (gr) (void)hipGraphDestroy(gr);
+            float ms = 0.f;
+            if (!rc) {
+                hipError_t e2 = hipGraphLaunch(ge, stream);   // warm-up
+                if (e2 == hipSuccess) e2 = hipEventRecord(ev[0], stream);
+                for (int r = 0; r < reps && e2 == hipSuccess; ++r) e2 = hipGraphLaunch(ge, stream);
+                if (e2 == hipSuccess) e2 = hipEventRecord(ev[1], stream);
+                if (e2 == hipSuccess) e2 = hipEventSynchronize(ev[1]);
+                if (e2 == hipSuccess) e2 = hipEventElapsedTime(&ms, ev[0], ev[1]);
+                if (e2 != hipSuccess) rc = fail(OPO_ERR_HIP, "stage timing: %s", hipGetErrorString(e2));
+            }
+            if (ge) (void)hipGraphExecDestroy(ge);
+            if (rc) break;
+            const double alg = key.first == OPO_STAGE_POWER
+                                   ? class_bytes(3, 1, [](const BatchEntry *) { return true; })
+                               : key.first == OPO_STAGE_XY_OP
+                                   ? class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nx == key.second; })
+                               : key.first == OPO_STAGE_FREE_SPACE
+                                   ? class_bytes(2, 2, [&](const BatchEntry *e) { return e->g.nx == key.second && e->p.fs_on; })
+                                   : class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nt == key.second; });
+            out.push_back(opo_stage_time{key.first, key.second, launches * reps, (double)ms, alg});
+        }
+        only_stage = only_size = -1;
+        profiling = keep_prof;
+        for (auto e : ev) (void)hipEventDestroy(e);
+        return rc;
+    }
 };
 
 extern "C" {
@@ -1603,6 +1731,29 @@ int opo_batch_poll(opo_batch *b, uint64_t ticket, int *done, double *power, void
     return OPO_OK;
 }
 
+int opo_batch_rebuilds(opo_batch *b, int *rebuilds, int *updated, double *seconds) {
+    if (!b) return fail(OPO_ERR_ARG, "null batch");
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (rebuilds) *rebuilds = b->n_rebuilds;
+    if (updated) *updated = b->n_updated;
+    if (seconds) *seconds = b->rebuild_s;
+    return OPO_OK;
+}
+
+int opo_batch_time_stages(opo_batch *b, int reps, opo_stage_time *out, int cap, int *n) {
+    if (!b || !n || reps < 1 || (cap > 0 && !out)) return fail(OPO_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    CK(hipSetDevice(b->device));
+    if (b->resident.empty()) return fail(OPO_ERR_STATE, "no resident solves to time");
+    int rc;
+    if (b->dirty && (rc = b->rebuild())) return rc;
+    std::vector<opo_stage_time> v;
+    if ((rc = b->time_stages(reps, v))) return rc;
+    *n = (int)v.size();
+    for (int k = 0; k < cap && k < *n; ++k) out[k] = v[k];
+    return OPO_OK;
+}
+
 int opo_batch_profile(opo_batch *b, int on) {
     if (!b) return fail(OPO_ERR_ARG, "null batch");
     std::lock_guard<std::mutex> lk(b->mu);
@@ -1611,6 +1762,7 @@ int opo_batch_profile(opo_batch *b, int on) {
         b->profiling = want;
         b->dirty = true;
         b->prof.clear();
+        b->prof_error.clear();
     }
     for (opo_stage_time &e : b->prof) {   // start the sums again
         e.launches = 0;
@@ -1622,6 +1774,7 @@ int opo_batch_profile(opo_batch *b, int on) {
 int opo_batch_profile_read(opo_batch *b, opo_stage_time *out, int cap, int *n) {
     if (!b || !n || (cap > 0 && !out)) return fail(OPO_ERR_ARG, "null argument");
     std::lock_guard<std::mutex> lk(b->mu);
+    if (!b->prof_error.empty()) return fail(OPO_ERR_HIP, "%s", b->prof_error.c_str());
     *n = (int)b->prof.size();
     for (int k = 0; k < cap && k < *n; ++k) out[k] = b->prof[k];
     return OPO_OK;

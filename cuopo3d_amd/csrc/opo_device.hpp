@@ -309,7 +309,25 @@ __device__ __forceinline__ cplx<T> table_at(const cplx<T> *__restrict__ p, int i
     }
 }
 
-template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false>
+// Register part of phase 1 (two-level split) for sub-index n2: N1-point
+// DFT of v[n1] = x[N2 n1 + n2], then the inter-level twiddles W_N^(n2 k1).
+template <typename T, int N, int S, bool UNI = false>
+__device__ __forceinline__ void phase1_regs(cplx<T> *v, int n2, const cplx<T> *__restrict__ tw) {
+    constexpr int N1 = Split<N>::N1;
+    dft<N1, S>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+        const int j = n2 * k1;
+        v[k1] = S < 0 ? cmul_hl(v[k1], table_at<T, UNI>(tw, 2 * j), table_at<T, UNI>(tw, 2 * j + 1))
+                      : cmulc_hl(v[k1], table_at<T, UNI>(tw, 2 * j), table_at<T, UNI>(tw, 2 * j + 1));
+    }
+}
+
+// P1 / P1P false: the caller does phase 1 (phase 1') itself, with
+// phase1_regs (an inverse N1-point DFT) on registers, and the LDS holds the
+// phase-1 output on entry (the phase-1' input on return); the operations of
+// every element are the same.
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false, bool P1 = true, bool P1P = true>
 __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
                                             const cplx<T> *__restrict__ tw, int tid, int nthr) {
     constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
@@ -334,23 +352,19 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
         __syncthreads();
     } else {
         // phase 1: task (l, n2, f)
-        for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
-            const int l = task % nlines, r = uni(task / nlines), n2 = r % N2, f = r / N2;
-            cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
-            cplx<T> v[N1];
+        if constexpr (P1) {
+            for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
+                const int l = task % nlines, r = uni(task / nlines), n2 = r % N2, f = r / N2;
+                cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
+                cplx<T> v[N1];
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[n1 * RS];
-            dft<N1, S>(v);
+                for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[n1 * RS];
+                phase1_regs<T, N, S, UNI>(v, n2, tw);
 #pragma unroll
-            for (int k1 = 1; k1 < N1; ++k1) {
-                const int j = n2 * k1;
-                v[k1] = S < 0 ? cmul_hl(v[k1], table_at<T, UNI>(tw, 2 * j), table_at<T, UNI>(tw, 2 * j + 1))
-                              : cmulc_hl(v[k1], table_at<T, UNI>(tw, 2 * j), table_at<T, UNI>(tw, 2 * j + 1));
+                for (int k1 = 0; k1 < N1; ++k1) base[k1 * RS] = v[k1];
             }
-#pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) base[k1 * RS] = v[k1];
+            __syncthreads();
         }
-        __syncthreads();
         // phase 2: task (l, k1, f)
         for (int task = tid; task < nlines * N1 * ls.nf; task += nthr) {
             const int l = task % nlines, r = uni(task / nlines), k1 = r % N1, f = r / N1;
@@ -377,17 +391,19 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
         }
         __syncthreads();
         // phase 1': task (l, n2, f)
-        for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
-            const int l = task % nlines, r = uni(task / nlines), n2 = r % N2, f = r / N2;
-            cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
-            cplx<T> v[N1];
+        if constexpr (P1P) {
+            for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
+                const int l = task % nlines, r = uni(task / nlines), n2 = r % N2, f = r / N2;
+                cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
+                cplx<T> v[N1];
 #pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[k1 * RS];
-            dft<N1, -S>(v);
+                for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[k1 * RS];
+                dft<N1, -S>(v);
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) base[n1 * RS] = v[n1];
+                for (int n1 = 0; n1 < N1; ++n1) base[n1 * RS] = v[n1];
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
@@ -415,7 +431,22 @@ template <int N> struct Split3 {
 //   A' per (l, m):      inverse N1-point DFT
 // W_M^j is the N-point twiddle W_N^(N1 j).  The same LineSet conventions
 // (fields, tables, task order line-fastest) as line_op_set.
-template <typename T, int N, int S, int LS, int ES, int PAD = 0>
+// Register part of phase A (three-level split) for sub-index m: N1-point
+// DFT of v[n1] = x[M n1 + m], then the twiddles W_N^(m k1).
+template <typename T, int N, int S>
+__device__ __forceinline__ void phaseA_regs(cplx<T> *v, int m, const cplx<T> *__restrict__ tw) {
+    constexpr int N1 = Split3<N>::N1;
+    dft<N1, S>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+        const int j = m * k1;
+        v[k1] = S < 0 ? cmul_hl(v[k1], tw[2 * j], tw[2 * j + 1]) : cmulc_hl(v[k1], tw[2 * j], tw[2 * j + 1]);
+    }
+}
+
+// PA / PAP false: the caller does phase A (phase A') itself, as with
+// line_op_set's P1 / P1P.
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool PA = true, bool PAP = true>
 __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
                                              const cplx<T> *__restrict__ tw, int tid, int nthr) {
     using Q = Split3<N>;
@@ -428,19 +459,19 @@ __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
         return S < 0 ? cmulc_hl(a, tw[2 * j], tw[2 * j + 1]) : cmul_hl(a, tw[2 * j], tw[2 * j + 1]);
     };
     // A: task (l, m, f)
-    for (int task = tid; task < nlines * M * ls.nf; task += nthr) {
-        const int l = task % nlines, r = task / nlines, m = r % M, f = r / M;
-        cplx<T> *base = ls.base(f) + l * LS;
-        cplx<T> v[N1];
+    if constexpr (PA) {
+        for (int task = tid; task < nlines * M * ls.nf; task += nthr) {
+            const int l = task % nlines, r = task / nlines, m = r % M, f = r / M;
+            cplx<T> *base = ls.base(f) + l * LS;
+            cplx<T> v[N1];
 #pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[pos(M * n1 + m)];
-        dft<N1, S>(v);
+            for (int n1 = 0; n1 < N1; ++n1) v[n1] = base[pos(M * n1 + m)];
+            phaseA_regs<T, N, S>(v, m, tw);
 #pragma unroll
-        for (int k1 = 1; k1 < N1; ++k1) v[k1] = fwd(v[k1], m * k1);
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) base[pos(M * k1 + m)] = v[k1];
+            for (int k1 = 0; k1 < N1; ++k1) base[pos(M * k1 + m)] = v[k1];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // B: task (l, n3, k1, f)
     for (int task = tid; task < nlines * N3 * N1 * ls.nf; task += nthr) {
         const int l = task % nlines, r = task / nlines, n3 = r % N3, r2 = r / N3, k1 = r2 % N1, f = r2 / N1;
@@ -494,36 +525,64 @@ __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
     }
     __syncthreads();
     // A': task (l, m, f)
-    for (int task = tid; task < nlines * M * ls.nf; task += nthr) {
-        const int l = task % nlines, r = task / nlines, m = r % M, f = r / M;
-        cplx<T> *base = ls.base(f) + l * LS;
-        cplx<T> v[N1];
+    if constexpr (PAP) {
+        for (int task = tid; task < nlines * M * ls.nf; task += nthr) {
+            const int l = task % nlines, r = task / nlines, m = r % M, f = r / M;
+            cplx<T> *base = ls.base(f) + l * LS;
+            cplx<T> v[N1];
 #pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[pos(M * k1 + m)];
-        dft<N1, -S>(v);
+            for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[pos(M * k1 + m)];
+            dft<N1, -S>(v);
 #pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1) base[pos(M * n1 + m)] = v[n1];
+            for (int n1 = 0; n1 < N1; ++n1) base[pos(M * n1 + m)] = v[n1];
+        }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
-// Lines of 2048 samples take the three-level operator: as N1 * N2 = 64 * 32
-// their first phase has 64-point register DFTs and a quarter of the tasks
-// of the other lengths (measured 1.8x slower per tile); at 512 and 1024 the
-// two-level operator is the faster one (by 3-4 %).
-template <int N> constexpr bool three_level() { return N == 2048; }
+// Which lines take the three-level operator (measured per slice, t pass
+// and xy pass together, on MI355X):
+//  * 2048 samples: as N1 * N2 = 64 * 32 the first phase would have 64-point
+//    register DFTs and a quarter of the tasks of the other lengths (1.8x
+//    slower per tile);
+//  * 1024 samples: the 32-point register DFTs of the two-level split hold
+//    about 250 VGPRs per lane in FP32 (two waves per SIMD); with 16-point
+//    DFTs the cavity runs 10-11 % faster at 32 x 32 and 128 x 128, FP32
+//    and FP64;
+//  * 512 samples in FP64 (3 % faster at 64 x 64); FP32 512-sample lines
+//    stay two-level (three levels: 3 % faster at 128 x 128, 4 % slower at
+//    64 x 64).
+template <typename T, int N> constexpr bool three_level() {
+    return N == 2048 || N == 1024 || (sizeof(T) == 8 && N == 512);
+}
 // Length of the innermost group of a line (the groups that padding
 // separates in LDS).
-template <int N> constexpr int inner_len() {
-    if constexpr (three_level<N>()) return Split3<N>::N3;
+template <typename T, int N> constexpr int inner_len() {
+    if constexpr (three_level<T, N>()) return Split3<N>::N3;
     else return Split<N>::N2;
 }
 
-template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false>
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false, bool OUTER = true>
 __device__ __forceinline__ void line_op_any(const LineSet<T> &ls, int nlines, const cplx<T> *__restrict__ tw,
                                             int tid, int nthr) {
-    if constexpr (three_level<N>()) line_op_set3<T, N, S, LS, ES, PAD>(ls, nlines, tw, tid, nthr);
-    else line_op_set<T, N, S, LS, ES, PAD, UNI>(ls, nlines, tw, tid, nthr);
+    if constexpr (three_level<T, N>()) line_op_set3<T, N, S, LS, ES, PAD, OUTER, OUTER>(ls, nlines, tw, tid, nthr);
+    else line_op_set<T, N, S, LS, ES, PAD, UNI, OUTER, OUTER>(ls, nlines, tw, tid, nthr);
+}
+
+// The outer level of a line operator (phase 1 / 1' of the two-level split,
+// A / A' of the three-level one): element n = (N / L) j + q of a line goes
+// through the L-point register DFT of sub-index q, L = outer_len.
+template <typename T, int N> __host__ __device__ constexpr int outer_len() {
+    if constexpr (three_level<T, N>()) return Split3<N>::N1;
+    else return Split<N>::N1;
+}
+template <typename T, int N, int S>
+__device__ __forceinline__ void outer_forward(cplx<T> *v, int q, const cplx<T> *__restrict__ tw) {
+    if constexpr (three_level<T, N>()) phaseA_regs<T, N, S>(v, q, tw);
+    else phase1_regs<T, N, S, false>(v, q, tw);
+}
+template <typename T, int N, int S> __device__ __forceinline__ void outer_inverse(cplx<T> *v) {
+    dft<outer_len<T, N>(), -S>(v);
 }
 
 // One field.

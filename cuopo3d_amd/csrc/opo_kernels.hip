@@ -135,13 +135,15 @@ hipError_t launch_tline(hipStream_t st, const Geometry &g, const FieldOps<T> &op
     }
 }
 
-template <typename T, int NT, int TX, int BS, bool PHASE>
+template <typename T, int NT, int TX, int BS, bool PHASE, bool FUSED = false>
 __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
                                                const cplx<T> *__restrict__ tw, const int *status) {
     if (status && *status) return;
     __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
     __shared__ cplx<T> twl[time_stage_twiddles<T, NT, TX>() ? 2 * NT : 1];
-    nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds, twl);
+    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
+    if constexpr (FUSED) nl_time_tile_fused<T, NT, TX, BS, PHASE>(a, nx, ny, tile, tw, lds, twl);
+    else nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, tile, tw, lds, twl);
 }
 
 template <typename T, int NT, int TX, int BS>
@@ -205,7 +207,7 @@ static int nx_index(int nx) {
 }
 
 template <typename T, int NT>
-static void time_variant_info(int v, bool &valid, int &tx, int &bs, int &per_cu) {
+static void time_variant_info(int v, bool &valid, int &tx, int &bs, int &per_cu, bool &fused) {
     valid = false;
     static_for<0, kTimeVariants>([&](auto vv) {
         constexpr int V = decltype(vv)::value;
@@ -215,15 +217,16 @@ static void time_variant_info(int v, bool &valid, int &tx, int &bs, int &per_cu)
             tx = Vr::tx;
             bs = Vr::bs;
             per_cu = Vr::per_cu;
+            fused = Vr::fused;
         }
     });
 }
 
 template <typename T>
-static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu) {
+static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu, bool &fused) {
     bool valid = false;
     switch (nt) {
-#define TI(N) case N: time_variant_info<T, N>(v, valid, tx, bs, per_cu); break;
+#define TI(N) case N: time_variant_info<T, N>(v, valid, tx, bs, per_cu, fused); break;
         TI(1) TI(16) TI(32) TI(64) TI(128) TI(256) TI(512) TI(1024) TI(2048)
 #undef TI
         default: return false;
@@ -232,9 +235,22 @@ static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu) {
 }
 
 template <typename T>
+static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu) {
+    bool fused = false;
+    return time_info<T>(nt, v, tx, bs, per_cu, fused);
+}
+
+template <typename T>
 static bool time_info(int nt, int v, int &tx, int &bs) {
     int per_cu = 0;
     return time_info<T>(nt, v, tx, bs, per_cu);
+}
+
+bool tune_fused(bool fp64, int nt, int v) {
+    int tx = 0, bs = 0, per_cu = 0;
+    bool fused = false;
+    const bool ok = fp64 ? time_info<double>(nt, v, tx, bs, per_cu, fused) : time_info<float>(nt, v, tx, bs, per_cu, fused);
+    return ok && fused;
 }
 
 int tune_variant_count(int axis) { return axis == 0 ? kTimeVariants : kXyVariants; }
@@ -319,8 +335,8 @@ static hipError_t tline3_launch_v(hipStream_t st, const Geometry &g, const TLine
             else
                 return hipErrorInvalidValue;
         } else {
-            hipLaunchKernelGGL((k_tline3<T, NT, Vr::tx, Vr::bs, false>), grid, dim3(Vr::bs), 0, st, a, g.nx,
-                               g.ny, tw, status);
+            hipLaunchKernelGGL((k_tline3<T, NT, Vr::tx, Vr::bs, false, Vr::fused>), grid, dim3(Vr::bs), 0, st, a,
+                               g.nx, g.ny, tw, status);
         }
         return hipGetLastError();
     }

@@ -53,6 +53,9 @@ class ServeReport:
     steps: int
     max_resident: int
     latencies: list = field(default_factory=list)
+    rebuilds: int = 0            # re-captures of the trip graph (resident set changed)
+    rebuilds_in_place: int = 0   # of which updated the instantiated graph in place
+    rebuild_s: float = 0.0       # host seconds they took
 
     @property
     def rate(self):
@@ -62,7 +65,9 @@ class ServeReport:
         return {"requests": self.n, "makespan_s": self.makespan_s,
                 "cell_steps_per_s": self.rate, "p50_latency_s": self.p50_s,
                 "p99_latency_s": self.p99_s, "mean_latency_s": self.mean_s,
-                "batch_steps": self.steps, "max_resident": self.max_resident}
+                "batch_steps": self.steps, "max_resident": self.max_resident,
+                "rebuilds": self.rebuilds, "rebuilds_in_place": self.rebuilds_in_place,
+                "rebuild_ms_mean": 1e3 * self.rebuild_s / self.rebuilds if self.rebuilds else 0.0}
 
 
 def percentile(xs, q):
@@ -128,12 +133,14 @@ def serve(requests, budget_bytes, precision=OPO_FP32, keep_fields=True, clock=ti
                 del outstanding[t]
     makespan = clock() - t0
     lat = [r.done_at - r.arrival for r in requests]
+    rebuilds = batch.rebuilds()
     batch.close()
     return ServeReport(n=len(requests), makespan_s=makespan,
                        cell_steps=float(sum(r.cell_steps for r in requests if r.error is None)),
                        p50_s=percentile(lat, 50), p99_s=percentile(lat, 99),
                        mean_s=float(np.mean(lat)) if lat else float("nan"), steps=steps,
-                       max_resident=max_res, latencies=lat)
+                       max_resident=max_res, latencies=lat, rebuilds=rebuilds[0],
+                       rebuilds_in_place=rebuilds[1], rebuild_s=rebuilds[2])
 
 
 def poisson_requests(n, rate_per_s, seed=5, grids=configs.SERVING_GRIDS, trips=(50, 300),

@@ -289,6 +289,13 @@ size_t opo_batch_footprint(const opo_grid *grid, int precision, int n_trips);
 int opo_batch_stats(opo_batch *b, int *resident, int *queued,
                     size_t *bytes_used);
 
+/* Re-captures of the trip graph since the batch was created (every change
+ * of the resident set causes one at the next step), how many of them
+ * updated the instantiated graph in place instead of instantiating a new
+ * one, and the host seconds they took in total.  No counterpart in the
+ * reference (one process per sweep point, src/cuOPO3D.sh:56-86). */
+int opo_batch_rebuilds(opo_batch *b, int *rebuilds, int *updated, double *seconds);
+
 /* Kernel timing of batched round trips (no counterpart in the reference,
  * which only prints wall time, Common.cuh:83-90).  With profiling on, the
  * trip graph is captured with a HIP event recorded before and after every
@@ -308,8 +315,16 @@ typedef struct {
                            sum reads only), over the class's solves         */
 } opo_stage_time;
 int opo_batch_profile(opo_batch *b, int on);
-/* Copies up to cap entries to out; *n receives the number of entries. */
+/* Copies up to cap entries to out; *n receives the number of entries.
+ * OPO_ERR_HIP when the runtime could not time the graph's event nodes. */
 int opo_batch_profile_read(opo_batch *b, opo_stage_time *out, int cap, int *n);
+/* Time each (stage, size class) of the current trip on its own: a graph of
+ * that class's launches of one round trip (trip order, trip arguments),
+ * replayed reps times between two HIP events on the batch stream.  One
+ * entry per class as above (launches = launches per trip x reps).  The
+ * resident solves' fields go through the timed launches as through extra
+ * slices, so time a batch whose results are not needed afterwards. */
+int opo_batch_time_stages(opo_batch *b, int reps, opo_stage_time *out, int cap, int *n);
 
 #ifdef __cplusplus
 }

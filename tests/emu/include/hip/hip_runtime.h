@@ -13,3 +13,8 @@ inline void __syncthreads() {}
 inline int __builtin_amdgcn_readfirstlane(int x) { return x; }
 typedef int hipError_t;
 typedef void *hipStream_t;
+// One workgroup of one thread: index 0 everywhere.
+struct EmuDim3 {
+    unsigned x = 0, y = 0, z = 0;
+};
+inline EmuDim3 threadIdx, blockIdx, blockDim, gridDim;
