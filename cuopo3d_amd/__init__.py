@@ -149,17 +149,20 @@ def autotune_grids(grids, precision=OPO_FP32, reps=10, device=0):
     """Tune the launch variants for a set of (NX, NY, NT) grids, e.g. the
     size classes of a sweep or a serving mix, before they are run.
 
-    The xy pass is tuned per NX; the t pass per NT on the widest grid of
-    that NT (it carries most of the work), last, so that later calls do not
-    overwrite it.  Returns the tuner's reports."""
+    The xy pass is tuned per NX on the grid of that NX with the most time
+    samples (the most planes: a batch holds many planes of each NX, and
+    small planes favour other launch shapes in bulk than a few alone); the
+    t pass per NT on the widest grid of that NT (it carries most of the
+    work), last, so that later calls do not overwrite it.  Returns the
+    tuner's reports."""
     from . import configs
     grids = sorted({(int(g[0]), int(g[1]), int(g[2])) for g in grids})
     widest = {}
     for nx, ny, nt in grids:
         widest[nt] = max(widest.get(nt, (0, 0, 0)), (nx, ny, nt))
     by_nx = {}
-    for g in grids:
-        by_nx.setdefault(g[0], g)
+    for g in grids:   # sorted: the last grid of each NX has the longest lines
+        by_nx[g[0]] = g
     order = [g for g in by_nx.values() if g not in widest.values()] + list(widest.values())
     reports = []
     for nx, ny, nt in order:

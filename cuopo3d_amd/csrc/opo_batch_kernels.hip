@@ -257,7 +257,14 @@ static hipError_t bx_launch_bs(int bs, hipStream_t st, int n_work, const SolveDe
     const dim3 g((unsigned)n_work);
     if (bs == 256) hipLaunchKernelGGL((kb_xy<T, NX, 256, P>), g, dim3(256), 0, st, sd, ts, status, work, fs);
     else if (bs == 512) hipLaunchKernelGGL((kb_xy<T, NX, 512, P>), g, dim3(512), 0, st, sd, ts, status, work, fs);
-    else hipLaunchKernelGGL((kb_xy<T, NX, 1024, P>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
+    else if (bs == 1024) hipLaunchKernelGGL((kb_xy<T, NX, 1024, P>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
+    else if constexpr (P == 1 && NX >= 64) {
+        if (bs == 384) hipLaunchKernelGGL((kb_xy<T, NX, 384, 1>), g, dim3(384), 0, st, sd, ts, status, work, fs);
+        else if (bs == 768) hipLaunchKernelGGL((kb_xy<T, NX, 768, 1>), g, dim3(768), 0, st, sd, ts, status, work, fs);
+        else return hipErrorInvalidValue;
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

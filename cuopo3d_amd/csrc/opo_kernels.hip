@@ -264,6 +264,7 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
     if (n < 16 || (fp64 && n == 128)) return v == 0;   // FP64 128: the two-pass form only
     if (n <= 32) return v < 6;
+    if (v >= 9) return true;   // 384 or 768 threads, one plane
     if (xy_variant_persistent(n, v)) return !fp64 && n * n / xy_variant_bs(n, v) <= 16;
     return true;
 }
@@ -466,6 +467,10 @@ static hipError_t xy_launch_bs(int bs, hipStream_t st, const Geometry &g, const 
                                const cplx<T> *tw, const int *status) {
     if (bs == 256) return xy_launch_groups<T, NX, 256, P>(st, g, ops, nf, tw, status);
     if (bs == 512) return xy_launch_groups<T, NX, 512, P>(st, g, ops, nf, tw, status);
+    if constexpr (P == 1 && NX >= 64) {
+        if (bs == 384) return xy_launch_groups<T, NX, 384, 1>(st, g, ops, nf, tw, status);
+        if (bs == 768) return xy_launch_groups<T, NX, 768, 1>(st, g, ops, nf, tw, status);
+    }
     return xy_launch_groups<T, NX, 1024, P>(st, g, ops, nf, tw, status);
 }
 

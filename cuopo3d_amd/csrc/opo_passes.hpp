@@ -96,9 +96,14 @@ constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 6
 // workgroup; 512 threads held to 80 registers so that three 64 x 64
 // planes share a CU; one line per thread on 64-256 threads for NX <= 32;
 // persistent workgroups for NX <= 32.)
-constexpr int kXyVariants = 9;
-constexpr bool xy_variant_persistent(int nx, int v) { return nx >= 64 && v >= 3; }
+// V = 9, 10 (NX >= 64): one plane per workgroup of 384 or 768 threads,
+// which balance the 96-point lines (12 x 8 split: 768 tasks in phase 1,
+// 1152 in phase 2).
+constexpr int kXyVariants = 11;
+constexpr bool xy_variant_persistent(int nx, int v) { return nx >= 64 && v >= 3 && v < 9; }
 constexpr int xy_variant_bs(int nx, int v) {
+    if (v == 9) return 384;
+    if (v == 10) return 768;
     const int b = xy_variant_persistent(nx, v) ? (v - 3) / 2 : v % 3;
     return b == 0 ? 256 : (b == 1 ? 512 : 1024);
 }
