@@ -57,8 +57,11 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the rocFFT comparator")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
     ap.add_argument("--tune-reps", type=int, default=3, help="timed stretches per tuning candidate")
+    ap.add_argument("--tune", choices=("batch", "grids"), default="batch",
+                    help="tune the sweep's variants on the batch's own trip, or each grid alone")
     ap.add_argument("--variants-out", help="write the launch variants in use to this JSON file")
     ap.add_argument("--variants-in", help="set the launch variants from this JSON file (no tuning)")
+    ap.add_argument("--variants", help="set launch variants inline instead of tuning, e.g. t512=7,xy128=7")
     ap.add_argument("--serve-overload", action="store_true",
                     help="also serve 16 requests at 10/s with 50-300 trips (about 50 s)")
     ap.add_argument("--horizon-trips", type=int, default=0,
@@ -69,6 +72,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 128 MiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
+    if args.variants:
+        args.variants_in = parse_variants(args.variants)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent solves shard across GPUs by footprint, no collectives).", file=sys.stderr)
@@ -128,6 +133,25 @@ def variants_get(lib, prec=0):
             "xy": {str(n): lib.opo_tune_get(prec, OPO_TUNE_XY_PASS, n) for n in NX_LIST}}
 
 
+def parse_variants(spec):
+    """'t512=7,xy128=7' -> {"t": {"512": 7}, "xy": {"128": 7}}"""
+    out = {"t": {}, "xy": {}}
+    for item in spec.split(","):
+        m = re.fullmatch(r"(t|xy)(\d+)=(\d+)", item.strip())
+        if not m:
+            raise SystemExit(f"bench.py: bad --variants item {item!r} (want t<NT>=<v> or xy<NX>=<v>)")
+        out[m.group(1)][m.group(2)] = int(m.group(3))
+    return out
+
+
+def load_variants(src):
+    return src if isinstance(src, dict) else json.load(open(src))
+
+
+def variants_tag(src):
+    return "inline --variants" if isinstance(src, dict) else "variants set from " + os.path.basename(src)
+
+
 def variants_set(lib, v, prec=0):
     from cuopo3d_amd.capi import OPO_TUNE_TIME_PASS, OPO_TUNE_XY_PASS, OPO_OK
     for axis, key in ((OPO_TUNE_TIME_PASS, "t"), (OPO_TUNE_XY_PASS, "xy")):
@@ -145,32 +169,49 @@ def sweep_headline(co, configs, lib, args, n_solves=64, trips=100):
     grids = sorted({(g.nx, g.ny, g.nt) for (g, _, _, _) in solves})
     tune = []
     log(f"sweep: {n_solves} solves set up")
+    work = sum(g.nx * g.ny * g.nt * p.n_slices for (g, p, _, _) in solves)   # cell-steps per trip
+    b = co.Batch(200 << 30)
+    tickets = [b.submit(g, p, *f, trips) for (g, p, _, f) in solves]
+    assert args.warmup + args.steps + 9 <= trips, "the timed trips must fit the solves' 100 trips"
+    tune_s = 0.0
     if args.variants_in:
-        variants_set(lib, json.load(open(args.variants_in)))
-        tune = ["variants set from " + os.path.basename(args.variants_in)]
+        variants_set(lib, load_variants(args.variants_in))
+        tune = [variants_tag(args.variants_in)]
     elif not args.no_tune:
         t0 = time.perf_counter()
-        for rep in co.autotune_grids(grids, reps=args.tune_reps):
-            tune += [ln.strip() for ln in rep.strip().splitlines()]
-        log(f"sweep autotune: {time.perf_counter() - t0:.1f} s")
+        if args.tune == "batch":   # admits the 64 solves, times each class's variants in the trip
+            tune = [ln.strip() for ln in b.autotune(reps=args.tune_reps).strip().splitlines()]
+        else:
+            for rep in co.autotune_grids(grids, reps=args.tune_reps):
+                tune += [ln.strip() for ln in rep.strip().splitlines()]
+        tune_s = time.perf_counter() - t0
+        log(f"sweep autotune ({args.tune}): {tune_s:.1f} s")
     variants = variants_get(lib)
     if args.variants_out:
         with open(args.variants_out, "w") as fh:
             json.dump(variants, fh)
-    work = sum(g.nx * g.ny * g.nt * p.n_slices for (g, p, _, _) in solves)   # cell-steps per trip
-    b = co.Batch(200 << 30)
-    tickets = [b.submit(g, p, *f, trips) for (g, p, _, f) in solves]
-    assert args.warmup + args.steps + 4 <= trips, "the timed trips must fit the solves' 100 trips"
     t0 = time.perf_counter()
-    b.step()   # admission, descriptors, graph capture, first trip
+    b.step()   # admission (if not tuned), descriptors, graph capture, first trip
     admit_s = time.perf_counter() - t0
     for _ in range(max(0, args.warmup - 1)):
         b.step()
-    # a few trips without events: the rate the events are checked against
-    t0 = time.perf_counter()
-    for _ in range(3):
-        b.step()
-    plain_rate = work * 3 / (time.perf_counter() - t0)
+    # a few trips without events, size classes of a stage one after another
+    # and then as parallel branches of the trip graph; the faster form runs
+    # the timed trips
+    def rate3():
+        t0 = time.perf_counter()
+        for _ in range(3):
+            b.step()
+        return work * 3 / (time.perf_counter() - t0)
+    serial_rate = rate3()
+    b.concurrent_classes(True)
+    b.step()   # re-capture with the branches
+    concurrent_rate = rate3()
+    concurrent = concurrent_rate > serial_rate
+    if not concurrent:
+        b.concurrent_classes(False)
+    plain_rate = max(serial_rate, concurrent_rate)
+    log(f"sweep: serial classes {serial_rate:.4g}, concurrent classes {concurrent_rate:.4g} cell-steps/s")
     b.profile(True)
     b.step()   # re-capture with the events (not timed)
     b.profile(True)   # sums back to zero
@@ -236,9 +277,12 @@ def sweep_headline(co, configs, lib, args, n_solves=64, trips=100):
         "kernel_timing": timing,
         "kernel_ms_per_step": round(kernel_ms_trip, 3),
         "rate_without_events": plain_rate,
-        "admission_s": round(admit_s, 3),
+        "classes": {"serial_rate": serial_rate, "concurrent_rate": concurrent_rate,
+                    "timed_with": "concurrent" if concurrent else "serial"},
+        "first_step_s": round(admit_s, 3),
         "variants": variants,
-        "autotune": tune,
+        "autotune": {"mode": variants_tag(args.variants_in) if args.variants_in else ("none" if args.no_tune else args.tune),
+                     "seconds": round(tune_s, 2), "report": tune},
     }
 
 
@@ -352,11 +396,15 @@ def single_pass_leg(co, configs):
     return summary
 
 
-def _batched_rate(co, solves, trips):
+def _batched_rate(co, solves, trips, concurrent=False, tune=False):
     work = sum(g.nx * g.ny * g.nt * p.n_slices for (g, p, _, _) in solves)
     b = co.Batch(200 << 30)
+    if concurrent:
+        b.concurrent_classes(True)
     for (g, p, _, f) in solves:
         b.submit(g, p, *f, trips + 2)
+    if tune:
+        b.autotune(reps=3)
     b.step()
     b.step()
     t0 = time.perf_counter()
@@ -393,11 +441,14 @@ def small_sweep_leg(co, configs, n_solves=48, trips=5):
         d = dict(d)
         d["nxy"], d["nt"] = sizes[k % len(sizes)]
         solves.append(configs.sweep_solve(d, n_round_trips=100))
-    co.autotune_grids([(g.nx, g.ny, g.nt) for (g, _, _, _) in solves], reps=3)
-    batched = _batched_rate(co, solves, trips)
+    serial = _batched_rate(co, solves, trips, tune=True)
+    concurrent = _batched_rate(co, solves, trips, concurrent=True)
+    batched = max(serial, concurrent)
+    co.autotune_grids(sorted({(g.nx, g.ny, g.nt) for (g, _, _, _) in solves}), reps=3)
     one = _one_at_a_time_rate(co, solves, trips)
     return {"solves": n_solves, "grids": ["32x32x128", "32x32x256", "64x64x128", "64x64x256"],
             "trips_timed": trips, "batched_cell_steps_per_s": batched,
+            "batched_serial_classes": serial, "batched_concurrent_classes": concurrent,
             "one_at_a_time_cell_steps_per_s": one, "speedup": batched / one}
 
 
@@ -405,6 +456,7 @@ def one_at_a_time_leg(co, configs, batched, trips=3):
     """The 64 sweep solves one after another through the single-solve API
     (fused kernels, trip graphs), against the batched headline."""
     solves = [configs.sweep_solve(d, n_round_trips=100) for d in configs.sweep_draws(n_solves=64, seed=3)]
+    co.autotune_grids(sorted({(g.nx, g.ny, g.nt) for (g, _, _, _) in solves}), reps=3)
     one = _one_at_a_time_rate(co, solves, trips)
     return {"trips_timed": trips, "cell_steps_per_s": one, "batched_speedup": batched / one}
 
@@ -518,7 +570,7 @@ def cavity_profile_mode(co, configs, lib, args):
     s = co.Solver(grid, p, device=0)
     s.set_fields(*fields)
     if args.variants_in:
-        variants_set(lib, json.load(open(args.variants_in)))
+        variants_set(lib, load_variants(args.variants_in))
     elif not args.no_tune:
         s.autotune(reps=args.tune_reps)
     if args.variants_out:

@@ -231,6 +231,17 @@ class Batch:
               "opo_batch_stats")
         return r.value, q.value, u.value
 
+    def concurrent_classes(self, on=True):
+        """Run the size classes of each stage as parallel branches of the trip graph."""
+        check(self.lib.opo_batch_concurrent_classes(self.h, 1 if on else 0), "opo_batch_concurrent_classes")
+
+    def autotune(self, reps=5):
+        """Pick the fastest launch variant of each size class on this batch's
+        own trip (opo_batch_autotune); the solves' fields are kept."""
+        buf = C.create_string_buffer(1 << 16)
+        check(self.lib.opo_batch_autotune(self.h, reps, buf, len(buf)), "opo_batch_autotune")
+        return buf.value.decode()
+
     def rebuilds(self):
         """(re-captures of the trip graph, of which updated in place, host seconds)."""
         n, u, sec = C.c_int(), C.c_int(), C.c_double()

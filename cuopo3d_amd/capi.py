@@ -166,6 +166,8 @@ def load():
         "opo_batch_footprint": (sz, [C.POINTER(Grid), ip, ip]),
         "opo_batch_stats": (ip, [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(sz)]),
         "opo_batch_profile": (ip, [vp, ip]),
+        "opo_batch_concurrent_classes": (ip, [vp, ip]),
+        "opo_batch_autotune": (ip, [vp, ip, C.c_char_p, sz]),
         "opo_batch_rebuilds": (ip, [vp, C.POINTER(ip), C.POINTER(ip), dp]),
         "opo_batch_profile_read": (ip, [vp, C.POINTER(StageTime), ip, C.POINTER(ip)]),
         "opo_batch_time_stages": (ip, [vp, ip, C.POINTER(StageTime), ip, C.POINTER(ip)]),

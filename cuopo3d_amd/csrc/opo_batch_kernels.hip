@@ -59,7 +59,7 @@ __device__ __forceinline__ TLine3Args<T> batch_time_args(const SolveDesc<T> &d, 
     return a;
 }
 
-template <typename T, int NT, int TX, int BS, bool FUSED = false>
+template <typename T, int NT, int TX, int BS>
 __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ sd,
                                               const int *__restrict__ status,
                                               const WorkItem *__restrict__ work, int mode) {
@@ -68,10 +68,7 @@ __global__ void __launch_bounds__(BS) kb_time(const SolveDesc<T> *__restrict__ s
     const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
     if (status[w.solve]) return;
     const SolveDesc<T> &d = sd[w.solve];
-    if constexpr (FUSED)
-        nl_time_tile_fused<T, NT, TX, BS, false>(batch_time_args<T, NT>(d, mode), d.nx, d.ny, w.item, d.twt, lds, twl);
-    else
-        nl_time_tile<T, NT, TX, BS, false>(batch_time_args<T, NT>(d, mode), d.nx, d.ny, w.item, d.twt, lds, twl);
+    nl_time_tile<T, NT, TX, BS, false>(batch_time_args<T, NT>(d, mode), d.nx, d.ny, w.item, d.twt, lds, twl);
 }
 
 // Work units of the fused pass over a work list: tile w.item of solve w.solve.
@@ -225,8 +222,8 @@ static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd
                     hipLaunchKernelGGL((kb_timep<T, NT, Vr::tx, Vr::bs>), dim3(persistent_grid(n_work, Vr::per_cu)),
                                        dim3(Vr::bs), 0, st, sd, status, work, n_work, mode);
                 else
-                    hipLaunchKernelGGL((kb_time<T, NT, Vr::tx, Vr::bs, Vr::fused>), dim3((unsigned)n_work),
-                                       dim3(Vr::bs), 0, st, sd, status, work, mode);
+                    hipLaunchKernelGGL((kb_time<T, NT, Vr::tx, Vr::bs>), dim3((unsigned)n_work), dim3(Vr::bs), 0,
+                                       st, sd, status, work, mode);
                 e = hipGetLastError();
             }
         }
@@ -257,14 +254,7 @@ static hipError_t bx_launch_bs(int bs, hipStream_t st, int n_work, const SolveDe
     const dim3 g((unsigned)n_work);
     if (bs == 256) hipLaunchKernelGGL((kb_xy<T, NX, 256, P>), g, dim3(256), 0, st, sd, ts, status, work, fs);
     else if (bs == 512) hipLaunchKernelGGL((kb_xy<T, NX, 512, P>), g, dim3(512), 0, st, sd, ts, status, work, fs);
-    else if (bs == 1024) hipLaunchKernelGGL((kb_xy<T, NX, 1024, P>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
-    else if constexpr (P == 1 && NX >= 64) {
-        if (bs == 384) hipLaunchKernelGGL((kb_xy<T, NX, 384, 1>), g, dim3(384), 0, st, sd, ts, status, work, fs);
-        else if (bs == 768) hipLaunchKernelGGL((kb_xy<T, NX, 768, 1>), g, dim3(768), 0, st, sd, ts, status, work, fs);
-        else return hipErrorInvalidValue;
-    } else {
-        return hipErrorInvalidValue;
-    }
+    else hipLaunchKernelGGL((kb_xy<T, NX, 1024, P>), g, dim3(1024), 0, st, sd, ts, status, work, fs);
     return hipGetLastError();
 }
 

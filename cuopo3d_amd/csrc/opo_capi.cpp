@@ -800,7 +800,6 @@ template <typename T> struct SolveT final : SolveImpl {
                 const int pc = tune_per_cu(fp64, axis, n, v);
                 char pers[32] = "";
                 if (pc) std::snprintf(pers, sizeof(pers), ", persistent %d/CU", pc);
-                else if (axis == 0 && tune_fused(fp64, n, v)) std::snprintf(pers, sizeof(pers), ", fused");
                 else if (axis == 1 && xy_group(n, v) > 1)
                     std::snprintf(pers, sizeof(pers), ", %d planes", xy_group(n, v));
                 std::snprintf(buf, sizeof(buf), " v%d(tile %d, block %d%s) %.2f;", v,
@@ -1033,6 +1032,24 @@ struct opo_batch {
     std::map<uint64_t, std::unique_ptr<BatchEntry>> all;
     std::mutex mu;
     hipStream_t stream = nullptr;
+    // Concurrent size classes (opo_batch_concurrent_classes): within each
+    // stage of the trip graph the launches of the different size classes
+    // are captured as parallel branches on side streams (the classes of a
+    // stage touch disjoint solves), joined before the next stage.
+    bool concurrent = false;
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
+    std::vector<hipEvent_t> sync_ev;   // fork/join events, distinct per use in one capture
+    size_t sync_used = 0;
+    int next_sync_event(hipEvent_t &e) {
+        if (sync_used >= sync_ev.size()) {
+            hipEvent_t x;
+            CK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+            sync_ev.push_back(x);
+        }
+        e = sync_ev[sync_used++];
+        return OPO_OK;
+    }
     // device-side descriptors, rebuilt when the resident set changes
     void *d_desc = nullptr;
     int *d_status = nullptr;
@@ -1086,7 +1103,7 @@ struct opo_batch {
         prof.push_back(opo_stage_time{stage, size, 0, 0.0, alg});
         return (int)prof.size() - 1;
     }
-    int mark(int stage, int size, double alg) {
+    int mark(int stage, int size, double alg, hipStream_t st) {
         if (!profiling) return OPO_OK;
         if (ev_used + 1 >= ev_pool.size()) {
             for (int k = 0; k < 64; ++k) {
@@ -1097,7 +1114,7 @@ struct opo_batch {
         }
         if (ev_used % 2 == 0) ev_entry.push_back(prof_entry(stage, size, alg));
         // recorded as an event node of the captured trip graph
-        CK(hipEventRecordWithFlags(ev_pool[ev_used++], stream, hipEventRecordExternal));
+        CK(hipEventRecordWithFlags(ev_pool[ev_used++], st, hipEventRecordExternal));
         return OPO_OK;
     }
     // After a step: add the elapsed time of every bracketed launch.
@@ -1135,6 +1152,9 @@ struct opo_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         if (graph) (void)hipGraphExecDestroy(graph);
         for (auto e : ev_pool) (void)hipEventDestroy(e);
+        for (auto e : sync_ev) (void)hipEventDestroy(e);
+        for (auto &x : side)
+            if (x) (void)hipStreamDestroy(x);
         for (auto &kv : all)
             if (kv.second->d_series) (void)hipFree(kv.second->d_series);
         all.clear();
@@ -1255,6 +1275,7 @@ struct opo_batch {
         // capture one round trip of every resident solve
         ev_used = 0;
         ev_entry.clear();
+        sync_used = 0;
         hipGraph_t gr = nullptr;
         CK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         int rc = fp64 ? enqueue_trip<double>() : enqueue_trip<float>();
@@ -1308,13 +1329,39 @@ struct opo_batch {
     template <typename T> int enqueue_trip() {
         const SolveDesc<T> *sd = static_cast<const SolveDesc<T> *>(d_desc);
         n_enqueued = 0;
-        auto launch = [&](int stage, int size, double alg, auto &&fn) -> int {
+        // one launch of a stage on stream st: the filter, and the profiling
+        // events around it
+        auto launch = [&](int stage, int size, double alg, hipStream_t st, auto &&fn) -> int {
             if (!wanted(stage, size)) return OPO_OK;
-            int rc = mark(stage, size, alg);
+            int rc = mark(stage, size, alg, st);
             if (rc) return rc;
-            CK(fn());
+            CK(fn(st));
             n_enqueued += 1;
-            return mark(stage, size, alg);
+            return mark(stage, size, alg, st);
+        };
+        // the launches of one stage, class c on stream c (side streams for
+        // c >= 1 when the classes run concurrently), joined at the end
+        auto stage_of = [&](int nclass, auto &&one) -> int {
+            const bool par = concurrent && only_stage < 0 && nclass > 1;
+            const int nside = par ? std::min(nclass - 1, (int)kSide) : 0;
+            hipEvent_t fork = nullptr;
+            int rc;
+            if (par) {
+                if ((rc = next_sync_event(fork))) return rc;
+                CK(hipEventRecord(fork, stream));
+                for (int k = 0; k < nside; ++k) CK(hipStreamWaitEvent(side[k], fork, 0));
+            }
+            for (int c = 0; c < nclass; ++c) {
+                const hipStream_t st = (par && c > 0) ? side[(c - 1) % kSide] : stream;
+                if ((rc = one(c, st))) return rc;
+            }
+            for (int k = 0; k < nside; ++k) {
+                hipEvent_t join = nullptr;
+                if ((rc = next_sync_event(join))) return rc;
+                CK(hipEventRecord(join, side[k]));
+                CK(hipStreamWaitEvent(stream, join, 0));
+            }
+            return OPO_OK;
         };
         std::vector<double> tb, xb, fb;
         for (const Range &r : trange) tb.push_back(class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nt == r.cls; }));
@@ -1324,38 +1371,38 @@ struct opo_batch {
         const double pb = class_bytes(3, 1, [](const BatchEntry *) { return true; });
         int rc;
         for (int k = 0; k < n_slices; ++k) {
-            for (size_t c = 0; c < trange.size(); ++c) {
-                const Range &r = trange[c];
-                if ((rc = launch(OPO_STAGE_TIME_OP, r.cls, tb[c], [&] {
-                         return launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off,
-                                                     k == 0 ? BATCH_FIRST : BATCH_MID, r.v);
-                     })))
-                    return rc;
-            }
-            for (size_t c = 0; c < xyrange.size(); ++c) {
-                const Range &r = xyrange[c];
-                if ((rc = launch(OPO_STAGE_XY_OP, r.cls, xb[c], [&] {
-                         return launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0, r.v);
-                     })))
-                    return rc;
-            }
-        }
-        for (size_t c = 0; c < trange.size(); ++c) {
-            const Range &r = trange[c];
-            if ((rc = launch(OPO_STAGE_GDD, r.cls, tb[c], [&] {
-                     return launch_batch_time<T>(stream, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST, r.v);
+            const int mode = k == 0 ? BATCH_FIRST : BATCH_MID;
+            if ((rc = stage_of((int)trange.size(), [&](int c, hipStream_t st) {
+                     const Range &r = trange[c];
+                     return launch(OPO_STAGE_TIME_OP, r.cls, tb[c], st, [&](hipStream_t s2) {
+                         return launch_batch_time<T>(s2, r.cls, r.n, sd, d_status, d_work + r.off, mode, r.v);
+                     });
+                 })))
+                return rc;
+            if ((rc = stage_of((int)xyrange.size(), [&](int c, hipStream_t st) {
+                     const Range &r = xyrange[c];
+                     return launch(OPO_STAGE_XY_OP, r.cls, xb[c], st, [&](hipStream_t s2) {
+                         return launch_batch_xy<T>(s2, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0, r.v);
+                     });
                  })))
                 return rc;
         }
-        for (size_t c = 0; c < fsrange.size(); ++c) {
-            const Range &r = fsrange[c];
-            if ((rc = launch(OPO_STAGE_FREE_SPACE, r.cls, fb[c], [&] {
-                     return launch_batch_xy<T>(stream, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 1, r.v);
-                 })))
-                return rc;
-        }
-        return launch(OPO_STAGE_POWER, 0, pb, [&] {
-            return launch_batch_power<T>(stream, prange.n, (int)resident.size(), sd, d_ts, d_status,
+        if ((rc = stage_of((int)trange.size(), [&](int c, hipStream_t st) {
+                 const Range &r = trange[c];
+                 return launch(OPO_STAGE_GDD, r.cls, tb[c], st, [&](hipStream_t s2) {
+                     return launch_batch_time<T>(s2, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST, r.v);
+                 });
+             })))
+            return rc;
+        if ((rc = stage_of((int)fsrange.size(), [&](int c, hipStream_t st) {
+                 const Range &r = fsrange[c];
+                 return launch(OPO_STAGE_FREE_SPACE, r.cls, fb[c], st, [&](hipStream_t s2) {
+                     return launch_batch_xy<T>(s2, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 1, r.v);
+                 });
+             })))
+            return rc;
+        return launch(OPO_STAGE_POWER, 0, pb, stream, [&](hipStream_t s2) {
+            return launch_batch_power<T>(s2, prange.n, (int)resident.size(), sd, d_ts, d_status,
                                          d_work + prange.off, d_partial, max_chunk);
         });
     }
@@ -1364,9 +1411,54 @@ struct opo_batch {
     // that class's launches of one round trip, in trip order and with the
     // trip's arguments, replayed reps times between two HIP events on the
     // batch stream.  The solves' fields go through these launches too.
+    // One (stage, size class) of the trip by itself: its launches of one
+    // round trip captured as a graph and replayed reps times (after one
+    // untimed replay) between two events on the batch stream.
+    int time_key(int stage, int size, int reps, float &ms, int &launches) {
+        ms = 0.f;
+        launches = 0;
+        const bool keep_prof = profiling;
+        profiling = false;
+        only_stage = stage;
+        only_size = size;
+        hipGraph_t gr = nullptr;
+        hipGraphExec_t ge = nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        int rc = OPO_OK;
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            rc = fail(OPO_ERR_HIP, "stage timing: capture failed");
+        } else {
+            rc = prec == OPO_FP64 ? enqueue_trip<double>() : enqueue_trip<float>();
+            launches = n_enqueued;
+            const hipError_t e = hipStreamEndCapture(stream, &gr);
+            if (!rc && e != hipSuccess) rc = fail(OPO_ERR_HIP, "stage timing capture: %s", hipGetErrorString(e));
+        }
+        only_stage = only_size = -1;
+        profiling = keep_prof;
+        if (!rc && hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess)
+            rc = fail(OPO_ERR_HIP, "stage timing: instantiate failed");
+        if (gr) (void)hipGraphDestroy(gr);
+        if (!rc) {
+            hipError_t e2 = hipEventCreate(&ev[0]);
+            if (e2 == hipSuccess) e2 = hipEventCreate(&ev[1]);
+            if (e2 == hipSuccess) e2 = hipGraphLaunch(ge, stream);   // warm-up
+            if (e2 == hipSuccess) e2 = hipEventRecord(ev[0], stream);
+            for (int r = 0; r < reps && e2 == hipSuccess; ++r) e2 = hipGraphLaunch(ge, stream);
+            if (e2 == hipSuccess) e2 = hipEventRecord(ev[1], stream);
+            if (e2 == hipSuccess) e2 = hipEventSynchronize(ev[1]);
+            if (e2 == hipSuccess) e2 = hipEventElapsedTime(&ms, ev[0], ev[1]);
+            if (e2 != hipSuccess) rc = fail(OPO_ERR_HIP, "stage timing: %s", hipGetErrorString(e2));
+        }
+        if (ge) (void)hipGraphExecDestroy(ge);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        return rc;
+    }
+
+    // Time each (stage, size class) of the trip by itself (time_key).
+    // The solves' fields go through these launches too.
     int time_stages(int reps, std::vector<opo_stage_time> &out) {
         out.clear();
-        const bool fp64 = prec == OPO_FP64;
         // the (stage, size) entries of the trip, in trip order
         std::vector<std::pair<int, int>> keys;
         for (const Range &r : trange) keys.push_back({OPO_STAGE_TIME_OP, r.cls});
@@ -1374,40 +1466,11 @@ struct opo_batch {
         for (const Range &r : trange) keys.push_back({OPO_STAGE_GDD, r.cls});
         for (const Range &r : fsrange) keys.push_back({OPO_STAGE_FREE_SPACE, r.cls});
         keys.push_back({OPO_STAGE_POWER, 0});
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        CK(hipEventCreate(&ev[0]));
-        CK(hipEventCreate(&ev[1]));
-        const bool keep_prof = profiling;
-        profiling = false;
-        int rc = OPO_OK;
         for (const auto &key : keys) {
-            only_stage = key.first;
-            only_size = key.second;
-            hipGraph_t gr = nullptr;
-            hipGraphExec_t ge = nullptr;
-            if (hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
-                rc = fail(OPO_ERR_HIP, "stage timing: capture failed");
-                break;
-            }
-            rc = fp64 ? enqueue_trip<double>() : enqueue_trip<float>();
-            const int launches = n_enqueued;
-            const hipError_t e = hipStreamEndCapture(stream, &gr);
-            if (!rc && e != hipSuccess) rc = fail(OPO_ERR_HIP, "stage timing capture: %s", hipGetErrorString(e));
-            if (!rc && hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess)
-                rc = fail(OPO_ERR_HIP, "stage timing: instantiate failed");
-            if (gr) (void)hipGraphDestroy(gr);
             float ms = 0.f;
-            if (!rc) {
-                hipError_t e2 = hipGraphLaunch(ge, stream);   // warm-up
-                if (e2 == hipSuccess) e2 = hipEventRecord(ev[0], stream);
-                for (int r = 0; r < reps && e2 == hipSuccess; ++r) e2 = hipGraphLaunch(ge, stream);
-                if (e2 == hipSuccess) e2 = hipEventRecord(ev[1], stream);
-                if (e2 == hipSuccess) e2 = hipEventSynchronize(ev[1]);
-                if (e2 == hipSuccess) e2 = hipEventElapsedTime(&ms, ev[0], ev[1]);
-                if (e2 != hipSuccess) rc = fail(OPO_ERR_HIP, "stage timing: %s", hipGetErrorString(e2));
-            }
-            if (ge) (void)hipGraphExecDestroy(ge);
-            if (rc) break;
+            int launches = 0;
+            const int rc = time_key(key.first, key.second, reps, ms, launches);
+            if (rc) return rc;
             const double alg = key.first == OPO_STAGE_POWER
                                    ? class_bytes(3, 1, [](const BatchEntry *) { return true; })
                                : key.first == OPO_STAGE_XY_OP
@@ -1417,10 +1480,114 @@ struct opo_batch {
                                    : class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nt == key.second; });
             out.push_back(opo_stage_time{key.first, key.second, launches * reps, (double)ms, alg});
         }
-        only_stage = only_size = -1;
-        profiling = keep_prof;
-        for (auto e : ev) (void)hipEventDestroy(e);
+        return OPO_OK;
+    }
+
+    // Tune the launch variants in this batch's own trip: for each NT class
+    // the t-pass variants whose tile divides the NX of every resident solve
+    // of the class, for each NX class the xy-pass variants, each timed as
+    // that class's launches of one trip over all its solves (time_key), the
+    // fastest kept in the process-wide tables.  The resident fields are
+    // saved first and put back after, so the solves continue from where
+    // they were; this needs free device memory of the size of those fields.
+    int autotune(int reps, std::string &report) {
+        const bool fp64 = prec == OPO_FP64;
+        int rc;
+        if (dirty && (rc = rebuild())) return rc;
+        size_t total = 0;
+        for (const BatchEntry *e : resident) total += 3 * e->s->field_bytes();
+        unsigned char *save = nullptr;
+        if (hipMalloc(&save, total) != hipSuccess)
+            return fail(OPO_ERR_NOMEM, "batch autotune: no room to save the resident fields (%zu bytes)", total);
+        size_t off = 0;
+        for (const BatchEntry *e : resident)
+            for (int f = 0; f < 3; ++f) {
+                const size_t fb = e->s->field_bytes();
+                if (hipMemcpyAsync(save + off, e->s->dev_ptr(f), fb, hipMemcpyDeviceToDevice, stream) != hipSuccess) {
+                    (void)hipFree(save);
+                    return fail(OPO_ERR_HIP, "batch autotune: saving the fields failed");
+                }
+                off += fb;
+            }
+        std::vector<int> nts, nxs;
+        for (const Range &r : trange) nts.push_back(r.cls);
+        for (const Range &r : xyrange) nxs.push_back(r.cls);
+        rc = OPO_OK;
+        char buf[160];
+        for (int axis = 0; axis < 2 && !rc; ++axis) {
+            for (int n : axis == 0 ? nts : nxs) {
+                const int stage = axis == 0 ? OPO_STAGE_TIME_OP : OPO_STAGE_XY_OP;
+                int best = tune_get(fp64, axis, n);
+                double best_us = 1e30;
+                report += axis == 0 ? "batch t pass NT=" : "batch xy pass NX=";
+                report += std::to_string(n) + " (us per launch):";
+                for (int v = 0; v < tune_variant_count(axis) && !rc; ++v) {
+                    if (!tune_variant_valid(fp64, axis, n, v)) continue;
+                    bool fits = true;
+                    if (axis == 0)
+                        for (const BatchEntry *e : resident)
+                            if (e->g.nt == n && (e->g.nx == 1 || e->g.nx % tune_tile_px(fp64, n, v) != 0)) fits = false;
+                    if (!fits) continue;
+                    tune_set(fp64, axis, n, v);
+                    if ((rc = rebuild())) break;
+                    float ms = 0.f;
+                    int launches = 0;
+                    if ((rc = time_key(stage, n, reps, ms, launches))) break;
+                    const double us = launches ? 1e3 * ms / ((double)launches * reps) : 1e30;
+                    std::snprintf(buf, sizeof(buf), " v%d %.2f;", v, us);
+                    report += buf;
+                    if (us < best_us) {
+                        best_us = us;
+                        best = v;
+                    }
+                }
+                tune_set(fp64, axis, n, best);
+                report += " -> v" + std::to_string(best) + "\n";
+                if (rc) break;
+            }
+        }
+        off = 0;
+        for (const BatchEntry *e : resident)
+            for (int f = 0; f < 3; ++f) {
+                const size_t fb = e->s->field_bytes();
+                if (hipMemcpyAsync(e->s->dev_ptr(f), save + off, fb, hipMemcpyDeviceToDevice, stream) != hipSuccess &&
+                    !rc)
+                    rc = fail(OPO_ERR_HIP, "batch autotune: restoring the fields failed");
+                off += fb;
+            }
+        const hipError_t es = hipStreamSynchronize(stream);
+        (void)hipFree(save);
+        if (!rc && es != hipSuccess) rc = fail(OPO_ERR_HIP, "batch autotune: %s", hipGetErrorString(es));
+        dirty = true;
         return rc;
+    }
+
+    // Admit queued solves in arrival order while the head of the queue fits
+    // the budget.
+    void admit() {
+        while (!queue.empty() && used + queue.front()->footprint <= budget) {
+            BatchEntry *e = queue.front();
+            std::unique_ptr<SolveImpl> s;
+            int rc = make_solve(&e->g, &e->p, device, s);
+            if (rc == OPO_ERR_NOMEM && !resident.empty()) break;  // wait for memory to free up
+            queue.pop_front();
+            if (!rc) rc = s->set_fields(e->api.data(), e->as.data(), e->ai.data());
+            if (!rc && hipMalloc(&e->d_series, sizeof(double) * 3 * (size_t)e->n_trips) != hipSuccess)
+                rc = fail(OPO_ERR_NOMEM, "series allocation failed");
+            if (rc) {
+                e->finished = true;
+                e->error = rc;
+                e->error_msg = g_err;
+                continue;
+            }
+            e->api = {};
+            e->as = {};
+            e->ai = {};
+            e->s = std::move(s);
+            used += e->footprint;
+            resident.push_back(e);
+            dirty = true;
+        }
     }
 };
 
@@ -1644,30 +1811,7 @@ int opo_batch_step(opo_batch *b, int *n_active) {
     std::lock_guard<std::mutex> lk(b->mu);
     CK(hipSetDevice(b->device));
     if (n_active) *n_active = 0;
-    // admit in arrival order while the head of the queue fits the budget
-    while (!b->queue.empty() && b->used + b->queue.front()->footprint <= b->budget) {
-        BatchEntry *e = b->queue.front();
-        std::unique_ptr<SolveImpl> s;
-        int rc = make_solve(&e->g, &e->p, b->device, s);
-        if (rc == OPO_ERR_NOMEM && !b->resident.empty()) break;  // wait for memory to free up
-        b->queue.pop_front();
-        if (!rc) rc = s->set_fields(e->api.data(), e->as.data(), e->ai.data());
-        if (!rc && hipMalloc(&e->d_series, sizeof(double) * 3 * (size_t)e->n_trips) != hipSuccess)
-            rc = fail(OPO_ERR_NOMEM, "series allocation failed");
-        if (rc) {
-            e->finished = true;
-            e->error = rc;
-            e->error_msg = g_err;
-            continue;
-        }
-        e->api = {};
-        e->as = {};
-        e->ai = {};
-        e->s = std::move(s);
-        b->used += e->footprint;
-        b->resident.push_back(e);
-        b->dirty = true;
-    }
+    b->admit();
     const int n = (int)b->resident.size();
     if (n == 0) return OPO_OK;
     int rc;
@@ -1738,6 +1882,36 @@ int opo_batch_rebuilds(opo_batch *b, int *rebuilds, int *updated, double *second
     if (updated) *updated = b->n_updated;
     if (seconds) *seconds = b->rebuild_s;
     return OPO_OK;
+}
+
+int opo_batch_concurrent_classes(opo_batch *b, int on) {
+    if (!b) return fail(OPO_ERR_ARG, "null batch");
+    std::lock_guard<std::mutex> lk(b->mu);
+    const bool want = on != 0;
+    if (want && !b->side[0]) {
+        CK(hipSetDevice(b->device));
+        for (auto &x : b->side) CK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+    }
+    if (want != b->concurrent) {
+        b->concurrent = want;
+        b->dirty = true;   // re-capture with or without the branches
+    }
+    return OPO_OK;
+}
+
+int opo_batch_autotune(opo_batch *b, int reps, char *report, size_t report_len) {
+    if (!b || reps < 1) return fail(OPO_ERR_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    CK(hipSetDevice(b->device));
+    b->admit();
+    if (b->resident.empty()) return fail(OPO_ERR_STATE, "no resident solves to tune on");
+    std::string rep;
+    const int rc = b->autotune(reps, rep);
+    if (report && report_len) {
+        std::strncpy(report, rep.c_str(), report_len - 1);
+        report[report_len - 1] = 0;
+    }
+    return rc;
 }
 
 int opo_batch_time_stages(opo_batch *b, int reps, opo_stage_time *out, int cap, int *n) {

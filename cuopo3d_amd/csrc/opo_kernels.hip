@@ -135,15 +135,13 @@ hipError_t launch_tline(hipStream_t st, const Geometry &g, const FieldOps<T> &op
     }
 }
 
-template <typename T, int NT, int TX, int BS, bool PHASE, bool FUSED = false>
+template <typename T, int NT, int TX, int BS, bool PHASE>
 __global__ void __launch_bounds__(BS) k_tline3(TLine3Args<T> a, int nx, int ny,
                                                const cplx<T> *__restrict__ tw, const int *status) {
     if (status && *status) return;
     __shared__ cplx<T> lds[3 * TimeTile<T, NT, TX>::FT];
     __shared__ cplx<T> twl[time_stage_twiddles<T, NT, TX>() ? 2 * NT : 1];
-    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
-    if constexpr (FUSED) nl_time_tile_fused<T, NT, TX, BS, PHASE>(a, nx, ny, tile, tw, lds, twl);
-    else nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, tile, tw, lds, twl);
+    nl_time_tile<T, NT, TX, BS, PHASE>(a, nx, ny, xcd_local_tile(blockIdx.x, gridDim.x), tw, lds, twl);
 }
 
 template <typename T, int NT, int TX, int BS>
@@ -207,7 +205,7 @@ static int nx_index(int nx) {
 }
 
 template <typename T, int NT>
-static void time_variant_info(int v, bool &valid, int &tx, int &bs, int &per_cu, bool &fused) {
+static void time_variant_info(int v, bool &valid, int &tx, int &bs, int &per_cu) {
     valid = false;
     static_for<0, kTimeVariants>([&](auto vv) {
         constexpr int V = decltype(vv)::value;
@@ -217,16 +215,15 @@ static void time_variant_info(int v, bool &valid, int &tx, int &bs, int &per_cu,
             tx = Vr::tx;
             bs = Vr::bs;
             per_cu = Vr::per_cu;
-            fused = Vr::fused;
         }
     });
 }
 
 template <typename T>
-static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu, bool &fused) {
+static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu) {
     bool valid = false;
     switch (nt) {
-#define TI(N) case N: time_variant_info<T, N>(v, valid, tx, bs, per_cu, fused); break;
+#define TI(N) case N: time_variant_info<T, N>(v, valid, tx, bs, per_cu); break;
         TI(1) TI(16) TI(32) TI(64) TI(128) TI(256) TI(512) TI(1024) TI(2048)
 #undef TI
         default: return false;
@@ -235,23 +232,12 @@ static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu, bool &fused)
 }
 
 template <typename T>
-static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu) {
-    bool fused = false;
-    return time_info<T>(nt, v, tx, bs, per_cu, fused);
-}
-
-template <typename T>
 static bool time_info(int nt, int v, int &tx, int &bs) {
     int per_cu = 0;
     return time_info<T>(nt, v, tx, bs, per_cu);
 }
 
-bool tune_fused(bool fp64, int nt, int v) {
-    int tx = 0, bs = 0, per_cu = 0;
-    bool fused = false;
-    const bool ok = fp64 ? time_info<double>(nt, v, tx, bs, per_cu, fused) : time_info<float>(nt, v, tx, bs, per_cu, fused);
-    return ok && fused;
-}
+
 
 int tune_variant_count(int axis) { return axis == 0 ? kTimeVariants : kXyVariants; }
 
@@ -264,7 +250,6 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
     if (n < 16 || (fp64 && n == 128)) return v == 0;   // FP64 128: the two-pass form only
     if (n <= 32) return v < 6;
-    if (v >= 9) return true;   // 384 or 768 threads, one plane
     if (xy_variant_persistent(n, v)) return !fp64 && n * n / xy_variant_bs(n, v) <= 16;
     return true;
 }
@@ -336,8 +321,8 @@ static hipError_t tline3_launch_v(hipStream_t st, const Geometry &g, const TLine
             else
                 return hipErrorInvalidValue;
         } else {
-            hipLaunchKernelGGL((k_tline3<T, NT, Vr::tx, Vr::bs, false, Vr::fused>), grid, dim3(Vr::bs), 0, st, a,
-                               g.nx, g.ny, tw, status);
+            hipLaunchKernelGGL((k_tline3<T, NT, Vr::tx, Vr::bs, false>), grid, dim3(Vr::bs), 0, st, a, g.nx,
+                               g.ny, tw, status);
         }
         return hipGetLastError();
     }
@@ -467,10 +452,6 @@ static hipError_t xy_launch_bs(int bs, hipStream_t st, const Geometry &g, const 
                                const cplx<T> *tw, const int *status) {
     if (bs == 256) return xy_launch_groups<T, NX, 256, P>(st, g, ops, nf, tw, status);
     if (bs == 512) return xy_launch_groups<T, NX, 512, P>(st, g, ops, nf, tw, status);
-    if constexpr (P == 1 && NX >= 64) {
-        if (bs == 384) return xy_launch_groups<T, NX, 384, 1>(st, g, ops, nf, tw, status);
-        if (bs == 768) return xy_launch_groups<T, NX, 768, 1>(st, g, ops, nf, tw, status);
-    }
     return xy_launch_groups<T, NX, 1024, P>(st, g, ops, nf, tw, status);
 }
 

@@ -105,9 +105,6 @@ int tune_tile_px(bool fp64, int nt, int v);   // pixels per tile of a t-pass var
 int time_variant_for(bool fp64, int nt, int nx);
 int tune_block(bool fp64, int axis, int n, int v);
 int tune_per_cu(bool fp64, int axis, int n, int v);
-// Whether t-pass variant v of NT is a fused one (outer line-operator level
-// in registers straight from and to HBM).
-bool tune_fused(bool fp64, int nt, int v);
 // Grid of a persistent launch: per_cu workgroups per CU of the current
 // device, at most one per unit.
 unsigned persistent_grid(int units, int per_cu);  // persistent workgroups per CU, 0: one unit each

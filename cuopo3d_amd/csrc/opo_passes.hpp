@@ -59,18 +59,18 @@ template <typename T, int NT, int TX> struct TimeTile {
 // staged so that 4-pixel tiles fit; for 2048-sample lines, 4- and 8-pixel
 // tiles with one field in LDS at a time and the other two held in
 // registers, 28-240 % slower, the held fields spilling beside the
-// three-level line operator.)
-// V >= 27: the one-tile variants again (V - 27 = 3 m + b as for V < 9) in
-// the fused form (nl_time_tile_fused), for lines whose outer register DFT
-// has at most 16 points.
+// three-level line operator; a fused form whose threads load their
+// outer-level samples of all three fields straight from HBM into
+// registers, take the pointwise step and the outer register DFT before the
+// first LDS write and store straight from the inverse outer DFT (two LDS
+// round trips and two barriers fewer, bitwise the same results): ~245
+// VGPRs, 7-20 % slower on 128- to 2048-sample lines.)
 constexpr int kTimeClassic = 9;
-constexpr int kTimeFused = kTimeClassic + 18;
-constexpr int kTimeVariants = kTimeFused + 9;
+constexpr int kTimeVariants = kTimeClassic + 18;
 template <typename T, int NT, int V> struct TimeVariant {
     static constexpr int txd = tline3_tx<T, NT>();
-    static constexpr bool fused = V >= kTimeFused;
-    static constexpr bool persistent = V >= kTimeClassic && !fused;
-    static constexpr int idx = fused ? V - kTimeFused : (persistent ? V - kTimeClassic : V);
+    static constexpr bool persistent = V >= kTimeClassic;
+    static constexpr int idx = persistent ? V - kTimeClassic : V;
     static constexpr int m = persistent ? idx / 6 : idx / 3;
     static constexpr int b = persistent ? (idx / 2) % 3 : idx % 3;
     static constexpr int per_cu = persistent ? idx % 2 + 1 : 0;
@@ -80,8 +80,7 @@ template <typename T, int NT, int V> struct TimeVariant {
     static constexpr int table_bytes = 8 * NT * (int)sizeof(cplx<T>);  // twiddles + 3 tables, hi/lo
     static constexpr bool valid =
         V == 0 || ((sizeof(T) == 4 || !persistent) && NT >= 64 && tx >= 2 && tx <= 32 && data_bytes <= 114688 &&
-                   (!persistent || (NT <= 256 && data_bytes + table_bytes <= 147456)) &&
-                   (!fused || outer_len<T, NT>() <= 16));
+                   (!persistent || (NT <= 256 && data_bytes + table_bytes <= 147456)));
 };
 // Untuned choice: the default tile with 512 threads where it exists.
 constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 64) ? 1 : 0; }
@@ -95,15 +94,11 @@ constexpr int time_default_variant(bool fp64, int nt) { return (!fp64 && nt >= 6
 // offered, measured no faster: all three fields of one time sample per
 // workgroup; 512 threads held to 80 registers so that three 64 x 64
 // planes share a CU; one line per thread on 64-256 threads for NX <= 32;
-// persistent workgroups for NX <= 32.)
-// V = 9, 10 (NX >= 64): one plane per workgroup of 384 or 768 threads,
-// which balance the 96-point lines (12 x 8 split: 768 tasks in phase 1,
-// 1152 in phase 2).
-constexpr int kXyVariants = 11;
-constexpr bool xy_variant_persistent(int nx, int v) { return nx >= 64 && v >= 3 && v < 9; }
+// persistent workgroups for NX <= 32; one plane on 384 or 768 threads,
+// which balance the 96-point lines' 768 and 1152 tasks per phase.)
+constexpr int kXyVariants = 9;
+constexpr bool xy_variant_persistent(int nx, int v) { return nx >= 64 && v >= 3; }
 constexpr int xy_variant_bs(int nx, int v) {
-    if (v == 9) return 384;
-    if (v == 10) return 768;
     const int b = xy_variant_persistent(nx, v) ? (v - 3) / 2 : v % 3;
     return b == 0 ? 256 : (b == 1 ? 512 : 1024);
 }
@@ -258,87 +253,6 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
 }
 
 
-// Fused form of nl_time_tile, the same operations on every element: the
-// outer level of the line operator (phase 1 / A) is done in registers
-// straight from HBM, and the inverse outer level (phase 1' / A') writes
-// straight to HBM.  A task (pixel l, sub-index q) loads the OUT_N cells
-// t = OUT_STRIDE j + q of its pixel for all three fields, takes the
-// pointwise step on each, and runs the outer DFT of every transformed
-// field before its first LDS write; fields below first_field are stored
-// as they are.  Against nl_time_tile this drops the LDS write of the
-// pointwise result, the phase-1 read, the phase-1' write, the LDS read
-// before the stores and two barriers.
-template <typename T, int NT, int TX, int BS, bool PHASE>
-__device__ __forceinline__ void nl_time_tile_fused(const TLine3Args<T> &a, int nx, int ny, int tile,
-                                                   const cplx<T> *__restrict__ tw, cplx<T> *lds, cplx<T> *twl) {
-    using L = TimeTile<T, NT, TX>;
-    constexpr int FT = L::FT, ON = outer_len<T, NT>(), OS = NT / ON;
-    constexpr bool STW = time_stage_twiddles<T, NT, TX>();
-    constexpr int TWP = STW ? (2 * NT + BS - 1) / BS : 1;
-    const int tid = threadIdx.x;
-    cplx<T> twr[TWP];
-    if constexpr (STW) {
-#pragma unroll
-        for (int j = 0; j < TWP; ++j)
-            if (tid + j * BS < 2 * NT) twr[j] = tw[tid + j * BS];
-    }
-    const int tiles_x = nx / TX;
-    const int y = tile / tiles_x, x0 = (tile % tiles_x) * TX;
-    const size_t plane = (size_t)nx * ny;
-    const size_t base = (size_t)y * nx + x0;
-    const Rk4Args<T> &rk = a.rk;
-    const int ff = a.first_field;
-    // outer level, forward, with the loads and the pointwise step: task (l, q)
-    for (int task = tid; task < TX * OS; task += BS) {
-        const int l = task % TX, q = task / TX;
-        cplx<T> v[3][ON];
-#pragma unroll
-        for (int j = 0; j < ON; ++j) {
-            const size_t gi = (size_t)(OS * j + q) * plane + base + l;
-            v[0][j] = a.psrc[gi];
-            v[1][j] = a.f[1][gi];
-            v[2][j] = a.f[2][gi];
-        }
-#pragma unroll
-        for (int j = 0; j < ON; ++j) {
-            Tri<T> A = {v[0][j], v[1][j], v[2][j]};
-            nl_point<T, PHASE>(rk, OS * j + q, A);
-            v[0][j] = A.p;
-            v[1][j] = A.s;
-            v[2][j] = A.i;
-        }
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            if (f < ff) {   // untransformed (the pump at a trip's end)
-#pragma unroll
-                for (int j = 0; j < ON; ++j) a.f[f][(size_t)(OS * j + q) * plane + base + l] = v[f][j];
-            } else {
-                outer_forward<T, NT, +1>(v[f], q, tw);
-#pragma unroll
-                for (int k = 0; k < ON; ++k) lds[f * FT + L::at((OS * k + q) * TX + l)] = v[f][k];
-            }
-        }
-    }
-    if constexpr (STW) {
-#pragma unroll
-        for (int j = 0; j < TWP; ++j)
-            if (tid + j * BS < 2 * NT) twl[tid + j * BS] = twr[j];
-    }
-    __syncthreads();
-    const LineSet<T> ls{lds + ff * FT, FT, a.tab0, a.tab_stride, 3 - ff};
-    line_op_any<T, NT, +1, 1, TX, L::PAD, false, false>(ls, TX, STW ? twl : tw, tid, BS);
-    // outer level, inverse, with the stores: task (l, q, f)
-    for (int task = tid; task < TX * OS * (3 - ff); task += BS) {
-        const int l = task % TX, r = task / TX, q = r % OS, f = ff + r / OS;
-        cplx<T> v[ON];
-#pragma unroll
-        for (int k = 0; k < ON; ++k) v[k] = lds[f * FT + L::at((OS * k + q) * TX + l)];
-        outer_inverse<T, NT, +1>(v);
-        cplx<T> *dst = f == 0 ? a.f[0] : (f == 1 ? a.f[1] : a.f[2]);
-#pragma unroll
-        for (int j = 0; j < ON; ++j) dst[(size_t)(OS * j + q) * plane + base + l] = v[j];
-    }
-}
 
 // Contiguous run [u0, u1) of the work units of this workgroup.
 __device__ __forceinline__ void unit_run(int units, int &u0, int &u1) {

@@ -289,6 +289,28 @@ size_t opo_batch_footprint(const opo_grid *grid, int precision, int n_trips);
 int opo_batch_stats(opo_batch *b, int *resident, int *queued,
                     size_t *bytes_used);
 
+/* Run the size classes of each stage of the trip concurrently: the
+ * launches of a stage (the fused t pass of each NT class, the xy pass of
+ * each NX class, ...) are captured as parallel branches of the trip graph
+ * on side streams of default priority, joined before the next stage.  The
+ * classes of one stage touch disjoint solves, so results are bitwise the
+ * same either way.  Off by default; takes effect at the next step. */
+int opo_batch_concurrent_classes(opo_batch *b, int on);
+
+/* Tune the launch variants on this batch's own trip (the batched
+ * counterpart of opo_autotune; the reference has no launch variants).
+ * Admits what the budget allows from the queue, then for each NT class
+ * times every t-pass variant whose tile divides the NX of each of the
+ * class's resident solves, and for each NX class every xy-pass variant:
+ * the class's launches of one round trip over all its solves, replayed
+ * reps times.  The fastest of each class is kept in the process-wide
+ * variant tables (opo_tune_get).  The resident fields are saved before and
+ * put back after, so the solves' results are unchanged (all variants are
+ * bitwise equal anyway); this needs free device memory of the size of the
+ * resident fields (OPO_ERR_NOMEM otherwise).  A per-class report is
+ * written to report (report_len bytes, may be NULL). */
+int opo_batch_autotune(opo_batch *b, int reps, char *report, size_t report_len);
+
 /* Re-captures of the trip graph since the batch was created (every change
  * of the resident set causes one at the next step), how many of them
  * updated the instantiated graph in place instead of instantiating a new

@@ -126,6 +126,7 @@ int main() {
     time_tile<256, 4, +1, 2>("t256_two_level");
     time_tile<512, 4, +1, 2>("t512_two_level");
     time_tile<1024, 2, +1, 2>("t1024_two_level");
+    time_tile<256, 4, -1, 3>("t256_three_level");
     time_tile<512, 4, +1, 3>("t512_three_level");
     time_tile<1024, 2, -1, 3>("t1024_three_level");
     time_tile<2048, 2, +1, 3>("t2048_three_level");

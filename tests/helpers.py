@@ -41,10 +41,12 @@ def batch_mix(n=4, trips=3, seed=3):
     return out
 
 
-def batch_run(solves, trips, budget=8 << 30):
+def batch_run(solves, trips, budget=8 << 30, concurrent=False):
     """Submit the solves to one batch, step until all finish; results in order."""
     import cuopo3d_amd as co
     b = co.Batch(budget)
+    if concurrent:
+        b.concurrent_classes(True)
     tickets = [b.submit(g, p, *f, trips) for (g, p, ph, f) in solves]
     results = {}
     for _ in range(10 * trips * len(solves) + 10):

@@ -6,6 +6,8 @@ multiply-add is an explicit fused operation (no compiler contraction), so
 every variant must give bitwise the same fields; the tuner must leave the
 solve's fields as it found them.
 """
+import re
+
 import numpy as np
 import pytest
 
@@ -254,3 +256,43 @@ def test_wide_tuned_tile_falls_back_on_narrow_grids(restore_tuning, require_gpu)
     for (pg, fg), (rf, rp) in zip(got_batch, ref):
         for a, c in zip(fg, rf):
             assert np.array_equal(a, c)
+
+
+def test_batch_autotune_keeps_fields(restore_tuning, require_gpu):
+    """opo_batch_autotune times each size class's variants on the batch's own
+    trip, before the first trip (it admits the queue itself) and between
+    trips: the fields and per-trip powers afterwards are bitwise those of a
+    batch that was never tuned (the resident fields are saved and put back),
+    and each NT and NX class of the mix has a report line whose pick is the
+    variant now in the process-wide table."""
+    lib = restore_tuning
+    solves = batch_mix(4, trips=3)
+    ref, _ = batch_run(solves, 3)
+    b = co.Batch(8 << 30)
+    tickets = [b.submit(g, p, *f, 3) for (g, p, ph, f) in solves]
+    reports = [b.autotune(reps=1)]
+    b.step()
+    reports.append(b.autotune(reps=1))
+    results = {}
+    for _ in range(10):
+        b.step()
+        for t in tickets:
+            if t not in results:
+                r = b.poll(t)
+                if r is not None:
+                    results[t] = r
+        if len(results) == len(tickets):
+            break
+    b.close()
+    for k, ((pr, fr), t) in enumerate(zip(ref, tickets)):
+        pg, fg = results[t]
+        for x, y in zip(fr, fg):
+            assert np.array_equal(x, y), k
+        assert np.array_equal(pr, pg), k
+    print(reports[-1], flush=True)
+    for axis, name, sizes in ((OPO_TUNE_TIME_PASS, "t", {g.nt for (g, _, _, _) in solves}),
+                              (OPO_TUNE_XY_PASS, "xy", {g.nx for (g, _, _, _) in solves})):
+        for n in sizes:
+            m = re.search(rf"batch {name} pass N[TX]={n} .*-> v(\d+)", reports[-1])
+            assert m, (name, n)
+            assert lib.opo_tune_get(OPO_FP32, axis, n) == int(m.group(1)), (name, n)
