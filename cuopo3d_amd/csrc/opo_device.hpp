@@ -412,9 +412,9 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
 // pixels still gives each thread a task in every phase, at the price of two
 // more LDS exchanges and one more pair of twiddle multiplies.
 template <int N> struct Split3 {
-    static constexpr int N1 = (N == 512) ? 8 : (N == 1024) ? 16 : (N == 2048) ? 16 : 0;
-    static constexpr int N2 = (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 16 : 0;
-    static constexpr int N3 = (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 8 : 0;
+    static constexpr int N1 = (N == 256) ? 8 : (N == 512) ? 8 : (N == 1024) ? 16 : (N == 2048) ? 16 : 0;
+    static constexpr int N2 = (N == 256) ? 8 : (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 16 : 0;
+    static constexpr int N3 = (N == 256) ? 4 : (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 8 : 0;
     static constexpr int M = N2 * N3;
     static_assert(N1 * N2 * N3 == N, "unsupported three-level line length");
 };
@@ -549,11 +549,12 @@ __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
 //    about 250 VGPRs per lane in FP32 (two waves per SIMD); with 16-point
 //    DFTs the cavity runs 10-11 % faster at 32 x 32 and 128 x 128, FP32
 //    and FP64;
-//  * 512 samples in FP64 (3 % faster at 64 x 64); FP32 512-sample lines
-//    stay two-level (three levels: 3 % faster at 128 x 128, 4 % slower at
-//    64 x 64).
+//  * 512 and 256 samples in FP64 (64 x 64 cavity 2-4 % faster, 32 x 32
+//    256-sample lines 1.4 %); FP64 128-sample lines stay two-level (three
+//    levels 8 * 4 * 4: 3 % faster at 64 x 64, 4 % slower at 96 x 96), and
+//    so do FP32 512-sample lines (three levels within +-4 % by grid).
 template <typename T, int N> constexpr bool three_level() {
-    return N == 2048 || N == 1024 || (sizeof(T) == 8 && N == 512);
+    return N == 2048 || N == 1024 || (sizeof(T) == 8 && (N == 512 || N == 256));
 }
 // Length of the innermost group of a line (the groups that padding
 // separates in LDS).

@@ -164,11 +164,13 @@ def test_fp64_reference_default_grid_two_slices(require_gpu):
     _check(s, o, 1e-11, "fp64 2 slices 128x128x2048")
 
 
-@pytest.mark.parametrize("n", [(16, 16, 2048), (32, 32, 1024), (96, 96, 64)], ids=lambda n: "x".join(map(str, n)))
+@pytest.mark.parametrize("n", [(16, 16, 2048), (32, 32, 1024), (32, 32, 256), (96, 96, 64)],
+                         ids=lambda n: "x".join(map(str, n)))
 def test_fp64_long_lines(n, require_gpu):
     """FP64 fused passes on the longest time lines (their line operators
-    keep part of their registers in scratch) and on the largest FP64 planes
-    (96 x 96, 149 KB of LDS per plane): one trip at 1e-11."""
+    keep part of their registers in scratch), on 256-sample lines (the
+    three-level 8 x 8 x 4 operator) and on the largest FP64 planes (96 x 96,
+    149 KB of LDS per plane): one trip at 1e-11."""
     grid, p, ph, fields = configs.cavity_64(precision=OPO_FP64, n=n, n_round_trips=1)
     s, o = _pair(grid, p, fields)
     pg = s.runOPO(1)
