@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--variants-out", help="write the launch variants in use to this JSON file")
     ap.add_argument("--variants-in", help="set the launch variants from this JSON file (no tuning)")
     ap.add_argument("--variants", help="set launch variants inline instead of tuning, e.g. t512=7,xy128=7")
+    ap.add_argument("--classes", choices=("both", "serial", "concurrent"), default="both",
+                    help="size classes of a stage one after another, as parallel graph branches, or "
+                         "measure both and time the faster (default)")
     ap.add_argument("--serve-overload", action="store_true",
                     help="also serve 16 requests at 10/s with 50-300 trips (about 50 s)")
     ap.add_argument("--horizon-trips", type=int, default=0,
@@ -203,15 +206,18 @@ def sweep_headline(co, configs, lib, args, n_solves=64, trips=100):
         for _ in range(3):
             b.step()
         return work * 3 / (time.perf_counter() - t0)
-    serial_rate = rate3()
-    b.concurrent_classes(True)
-    b.step()   # re-capture with the branches
-    concurrent_rate = rate3()
-    concurrent = concurrent_rate > serial_rate
+    serial_rate = concurrent_rate = None
+    if args.classes != "concurrent":
+        serial_rate = rate3()
+    if args.classes != "serial":
+        b.concurrent_classes(True)
+        b.step()   # re-capture with the branches
+        concurrent_rate = rate3()
+    concurrent = args.classes == "concurrent" or (args.classes == "both" and concurrent_rate > serial_rate)
     if not concurrent:
         b.concurrent_classes(False)
-    plain_rate = max(serial_rate, concurrent_rate)
-    log(f"sweep: serial classes {serial_rate:.4g}, concurrent classes {concurrent_rate:.4g} cell-steps/s")
+    plain_rate = max(r for r in (serial_rate, concurrent_rate) if r is not None)
+    log(f"sweep: serial classes {serial_rate}, concurrent classes {concurrent_rate} cell-steps/s")
     b.profile(True)
     b.step()   # re-capture with the events (not timed)
     b.profile(True)   # sums back to zero

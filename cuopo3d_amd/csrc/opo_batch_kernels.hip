@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256) kb_power_final(const SolveDesc<T> *__rest
 bool batch_supported(int nx, int ny, int nt, bool fp64) {
     if (nx != ny || !tline_supported(nt, fp64) || nt < 16) return false;
     switch (nx) {
-        case 16: case 32: case 64: case 96: return true;
+        case 1: case 16: case 32: case 64: case 96: return true;
         case 128: return !fp64;
         default: return false;
     }
@@ -212,6 +212,11 @@ bool batch_supported(int nx, int ny, int nt, bool fp64) {
 template <typename T, int NT>
 static hipError_t bt_dispatch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const int *status,
                               const WorkItem *work, int mode, int v) {
+    if (v == kBatchPlaneVariant) {   // plane-wave solves: one pixel per workgroup
+        hipLaunchKernelGGL((kb_time<T, NT, 1, 256>), dim3((unsigned)n_work), dim3(256), 0, st, sd, status, work,
+                           mode);
+        return hipGetLastError();
+    }
     hipError_t e = hipErrorInvalidValue;
     static_for<0, kTimeVariants>([&](auto vv) {
         constexpr int V = decltype(vv)::value;

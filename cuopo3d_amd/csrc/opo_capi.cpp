@@ -1059,9 +1059,13 @@ struct opo_batch {
     struct Range {
         int cls, off, n;
         int v;   // launch variant the class's work list was built for
+        // Work items of slice k: the class's solves are listed by slice
+        // count, most first, so slice k runs the prefix nk[k] of the list
+        // (the solves with more than k slices).
+        std::vector<int> nk;
     };
     std::vector<Range> trange, xyrange, fsrange;
-    Range prange{0, 0, 0, 0};
+    Range prange{0, 0, 0, 0, {}};
     int n_slices = -1, max_chunk = 1;
     hipGraphExec_t graph = nullptr;
     bool dirty = true;
@@ -1070,13 +1074,19 @@ struct opo_batch {
     // Variant of the fused pass over NT class nt: the tuned one when its
     // tile divides the NX of every resident solve of the class, else
     // variant 0 (its tile divides every supported NX).
-    int class_time_variant(int nt) const {
+    int class_time_variant(int cls) const {
+        if (cls < 0) return kBatchPlaneVariant;
         const bool fp64 = prec == OPO_FP64;
-        const int v = tune_get(fp64, 0, nt);
+        const int v = tune_get(fp64, 0, cls);
         for (const BatchEntry *e : resident)
-            if (e->g.nt == nt && time_variant_for(fp64, nt, e->g.nx) != v) return 0;
+            if (tclass(e) == cls && time_variant_for(fp64, cls, e->g.nx) != v) return 0;
         return v;
     }
+    // Size class of a solve's t pass: NT, or -NT for plane-wave solves
+    // (NX = NY = 1: one pixel per workgroup, kBatchPlaneVariant).
+    static int tclass(const BatchEntry *e) { return e->g.nx == 1 ? -e->g.nt : e->g.nt; }
+    // Whether a solve takes the xy pass of class nx (plane-wave solves skip it).
+    static bool in_xy(const BatchEntry *e, int nx) { return e->g.nx == nx && !e->p.no_diffraction; }
     std::vector<int> current_variants() const {
         std::vector<int> v;
         const bool fp64 = prec == OPO_FP64;
@@ -1094,6 +1104,8 @@ struct opo_batch {
     std::vector<int> ev_entry;    // entry index of the pair (2k, 2k + 1)
     size_t ev_used = 0;
     std::vector<opo_stage_time> prof;
+    std::vector<double> prof_alg;   // algorithmic bytes summed over the timed launches of an entry
+    std::vector<double> ev_alg;     // algorithmic bytes of the launch of pair k
     int prof_entry(int stage, int size, double alg) {
         for (size_t k = 0; k < prof.size(); ++k)
             if (prof[k].stage == stage && prof[k].size == size) {
@@ -1101,7 +1113,12 @@ struct opo_batch {
                 return (int)k;
             }
         prof.push_back(opo_stage_time{stage, size, 0, 0.0, alg});
+        prof_alg.push_back(0.0);
         return (int)prof.size() - 1;
+    }
+    void prof_reset() {
+        for (opo_stage_time &x : prof) x.launches = 0, x.ms = 0.0;
+        prof_alg.assign(prof.size(), 0.0);
     }
     int mark(int stage, int size, double alg, hipStream_t st) {
         if (!profiling) return OPO_OK;
@@ -1112,7 +1129,10 @@ struct opo_batch {
                 ev_pool.push_back(e);
             }
         }
-        if (ev_used % 2 == 0) ev_entry.push_back(prof_entry(stage, size, alg));
+        if (ev_used % 2 == 0) {
+            ev_entry.push_back(prof_entry(stage, size, alg));
+            ev_alg.push_back(alg);
+        }
         // recorded as an event node of the captured trip graph
         CK(hipEventRecordWithFlags(ev_pool[ev_used++], st, hipEventRecordExternal));
         return OPO_OK;
@@ -1129,9 +1149,10 @@ struct opo_batch {
             if (e != hipSuccess) {
                 prof_error = std::string("timing the events of the trip graph: ") + hipGetErrorString(e);
                 (void)hipGetLastError();
-                for (opo_stage_time &x : prof) x.launches = 0, x.ms = 0.0;
+                prof_reset();
                 return OPO_OK;
             }
+            prof_alg[ev_entry[k / 2]] += ev_alg[k / 2];
             opo_stage_time &x = prof[ev_entry[k / 2]];
             x.ms += t;
             x.launches += 1;
@@ -1213,16 +1234,32 @@ struct opo_batch {
         fsrange.clear();
         const bool fp64 = prec == OPO_FP64;
         std::map<int, std::vector<int>> by_nt, by_nx;
+        n_slices = 0;
         for (int i = 0; i < n; ++i) {
-            by_nt[resident[i]->g.nt].push_back(i);
-            by_nx[resident[i]->g.nx].push_back(i);
+            by_nt[tclass(resident[i])].push_back(i);
+            if (!resident[i]->p.no_diffraction) by_nx[resident[i]->g.nx].push_back(i);
+            n_slices = std::max(n_slices, resident[i]->p.n_slices);
         }
+        // within a class, solves with more slices first (stable)
+        auto by_slices = [&](std::vector<int> &v) {
+            std::stable_sort(v.begin(), v.end(), [&](int a, int b) {
+                return resident[a]->p.n_slices > resident[b]->p.n_slices;
+            });
+        };
+        // per-slice prefix lengths of a class's work list
+        auto prefixes = [&](Range &r, const std::vector<std::pair<int, int>> &items) {
+            r.nk.assign(n_slices, 0);
+            for (const auto &si : items)   // (slices, work items) per solve
+                for (int k = 0; k < si.first; ++k) r.nk[k] += si.second;
+        };
         // Each class's variant is read once here: the work list and the
         // launches captured below both follow it, whatever the process-wide
         // choice becomes meanwhile (a change makes the next step rebuild).
         for (auto &kv : by_nt) {
-            Range r{kv.first, (int)work.size(), 0, class_time_variant(kv.first)};
-            const int tx = tune_tile_px(fp64, kv.first, r.v);
+            by_slices(kv.second);
+            Range r{kv.first, (int)work.size(), 0, class_time_variant(kv.first), {}};
+            const int tx = kv.first < 0 ? 1 : tune_tile_px(fp64, kv.first, r.v);
+            std::vector<std::pair<int, int>> items;
             for (int i : kv.second) {
                 const opo_grid &g = resident[i]->g;
                 if (tx == 0 || g.nx % tx != 0)
@@ -1230,31 +1267,37 @@ struct opo_batch {
                                 tx, g.nx, kv.first);
                 const int tiles = (g.nx / tx) * g.ny;
                 for (int t = 0; t < tiles; ++t) work.push_back(WorkItem{i, t});
+                items.push_back({resident[i]->p.n_slices, tiles});
             }
             r.n = (int)work.size() - r.off;
+            prefixes(r, items);
             trange.push_back(r);
         }
         for (auto &kv : by_nx) {
+            by_slices(kv.second);
             // one item per group of consecutive planes of a field (xy_group)
             const int xv = tune_get(fp64, 1, kv.first);
             const int pg = xy_group(kv.first, xv);
-            Range r{kv.first, (int)work.size(), 0, xv};
+            Range r{kv.first, (int)work.size(), 0, xv, {}};
+            std::vector<std::pair<int, int>> items;
             for (int i : kv.second) {
                 if (resident[i]->g.nt % pg != 0)
                     return fail(OPO_ERR_UNSUPPORTED, "NT=%d is not a multiple of the %d-plane xy group (NX=%d)",
                                 resident[i]->g.nt, pg, kv.first);
                 for (int it = 0; it < 3 * resident[i]->g.nt / pg; ++it) work.push_back(WorkItem{i, it});
+                items.push_back({resident[i]->p.n_slices, 3 * resident[i]->g.nt / pg});
             }
             r.n = (int)work.size() - r.off;
+            prefixes(r, items);
             xyrange.push_back(r);
-            Range q{kv.first, (int)work.size(), 0, xv};
+            Range q{kv.first, (int)work.size(), 0, xv, {}};
             for (int i : kv.second)
-                if (resident[i]->p.fs_on)
+                if (resident[i]->p.fs_on)   // in the NX class: not a plane-wave solve
                     for (int it = 0; it < 2 * resident[i]->g.nt / pg; ++it) work.push_back(WorkItem{i, it});
             q.n = (int)work.size() - q.off;
             if (q.n) fsrange.push_back(q);
         }
-        prange = Range{0, (int)work.size(), 0, 0};
+        prange = Range{0, (int)work.size(), 0, 0, {}};
         for (int i = 0; i < n; ++i)
             for (int c = 0; c < chunks_for(resident[i]->g); ++c) work.push_back(WorkItem{i, c});
         prange.n = (int)work.size() - prange.off;
@@ -1275,6 +1318,7 @@ struct opo_batch {
         // capture one round trip of every resident solve
         ev_used = 0;
         ev_entry.clear();
+        ev_alg.clear();
         sync_used = 0;
         hipGraph_t gr = nullptr;
         CK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
@@ -1363,26 +1407,35 @@ struct opo_batch {
             }
             return OPO_OK;
         };
-        std::vector<double> tb, xb, fb;
-        for (const Range &r : trange) tb.push_back(class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nt == r.cls; }));
-        for (const Range &r : xyrange) xb.push_back(class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nx == r.cls; }));
+        std::vector<double> tb, fb;
+        for (const Range &r : trange) tb.push_back(class_bytes(3, 2, [&](const BatchEntry *e) { return tclass(e) == r.cls; }));
+        // bytes of slice k's launch: the class's solves with more than k slices
+        auto slice_bytes = [&](bool t_pass, int cls, int k) {
+            return class_bytes(3, 2, [&](const BatchEntry *e) {
+                return (t_pass ? tclass(e) == cls : in_xy(e, cls)) && e->p.n_slices > k;
+            });
+        };
         for (const Range &r : fsrange)
-            fb.push_back(class_bytes(2, 2, [&](const BatchEntry *e) { return e->g.nx == r.cls && e->p.fs_on; }));
+            fb.push_back(class_bytes(2, 2, [&](const BatchEntry *e) { return in_xy(e, r.cls) && e->p.fs_on; }));
         const double pb = class_bytes(3, 1, [](const BatchEntry *) { return true; });
         int rc;
         for (int k = 0; k < n_slices; ++k) {
             const int mode = k == 0 ? BATCH_FIRST : BATCH_MID;
             if ((rc = stage_of((int)trange.size(), [&](int c, hipStream_t st) {
                      const Range &r = trange[c];
-                     return launch(OPO_STAGE_TIME_OP, r.cls, tb[c], st, [&](hipStream_t s2) {
-                         return launch_batch_time<T>(s2, r.cls, r.n, sd, d_status, d_work + r.off, mode, r.v);
+                     const int nw = r.nk[k];
+                     if (nw == 0) return (int)OPO_OK;   // every solve of the class has fewer slices
+                     return launch(OPO_STAGE_TIME_OP, r.cls, slice_bytes(true, r.cls, k), st, [&](hipStream_t s2) {
+                         return launch_batch_time<T>(s2, std::abs(r.cls), nw, sd, d_status, d_work + r.off, mode, r.v);
                      });
                  })))
                 return rc;
             if ((rc = stage_of((int)xyrange.size(), [&](int c, hipStream_t st) {
                      const Range &r = xyrange[c];
-                     return launch(OPO_STAGE_XY_OP, r.cls, xb[c], st, [&](hipStream_t s2) {
-                         return launch_batch_xy<T>(s2, r.cls, r.n, sd, d_ts, d_status, d_work + r.off, 0, r.v);
+                     const int nw = r.nk[k];
+                     if (nw == 0) return (int)OPO_OK;
+                     return launch(OPO_STAGE_XY_OP, r.cls, slice_bytes(false, r.cls, k), st, [&](hipStream_t s2) {
+                         return launch_batch_xy<T>(s2, r.cls, nw, sd, d_ts, d_status, d_work + r.off, 0, r.v);
                      });
                  })))
                 return rc;
@@ -1390,7 +1443,8 @@ struct opo_batch {
         if ((rc = stage_of((int)trange.size(), [&](int c, hipStream_t st) {
                  const Range &r = trange[c];
                  return launch(OPO_STAGE_GDD, r.cls, tb[c], st, [&](hipStream_t s2) {
-                     return launch_batch_time<T>(s2, r.cls, r.n, sd, d_status, d_work + r.off, BATCH_LAST, r.v);
+                     return launch_batch_time<T>(s2, std::abs(r.cls), r.n, sd, d_status, d_work + r.off, BATCH_LAST,
+                                                 r.v);
                  });
              })))
             return rc;
@@ -1471,13 +1525,26 @@ struct opo_batch {
             int launches = 0;
             const int rc = time_key(key.first, key.second, reps, ms, launches);
             if (rc) return rc;
+            // per-slice stages: the mean over the trip's launches (solves
+            // with fewer slices drop out of the later launches)
+            auto per_slice = [&](bool t_pass) {
+                double sum = 0.0;
+                int nl = 0;
+                for (int k = 0; k < n_slices; ++k) {
+                    const double b = class_bytes(3, 2, [&](const BatchEntry *e) {
+                        return (t_pass ? tclass(e) == key.second : in_xy(e, key.second)) && e->p.n_slices > k;
+                    });
+                    if (b > 0) sum += b, nl += 1;
+                }
+                return nl ? sum / nl : 0.0;
+            };
             const double alg = key.first == OPO_STAGE_POWER
                                    ? class_bytes(3, 1, [](const BatchEntry *) { return true; })
-                               : key.first == OPO_STAGE_XY_OP
-                                   ? class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nx == key.second; })
+                               : key.first == OPO_STAGE_XY_OP ? per_slice(false)
                                : key.first == OPO_STAGE_FREE_SPACE
-                                   ? class_bytes(2, 2, [&](const BatchEntry *e) { return e->g.nx == key.second && e->p.fs_on; })
-                                   : class_bytes(3, 2, [&](const BatchEntry *e) { return e->g.nt == key.second; });
+                                   ? class_bytes(2, 2, [&](const BatchEntry *e) { return in_xy(e, key.second) && e->p.fs_on; })
+                               : key.first == OPO_STAGE_TIME_OP ? per_slice(true)
+                                   : class_bytes(3, 2, [&](const BatchEntry *e) { return tclass(e) == key.second; });
             out.push_back(opo_stage_time{key.first, key.second, launches * reps, (double)ms, alg});
         }
         return OPO_OK;
@@ -1510,7 +1577,8 @@ struct opo_batch {
                 off += fb;
             }
         std::vector<int> nts, nxs;
-        for (const Range &r : trange) nts.push_back(r.cls);
+        for (const Range &r : trange)
+            if (r.cls > 0) nts.push_back(r.cls);   // the plane-wave classes have one launch shape
         for (const Range &r : xyrange) nxs.push_back(r.cls);
         rc = OPO_OK;
         char buf[160];
@@ -1526,7 +1594,7 @@ struct opo_batch {
                     bool fits = true;
                     if (axis == 0)
                         for (const BatchEntry *e : resident)
-                            if (e->g.nt == n && (e->g.nx == 1 || e->g.nx % tune_tile_px(fp64, n, v) != 0)) fits = false;
+                            if (tclass(e) == n && e->g.nx % tune_tile_px(fp64, n, v) != 0) fits = false;
                     if (!fits) continue;
                     tune_set(fp64, axis, n, v);
                     if ((rc = rebuild())) break;
@@ -1781,14 +1849,11 @@ int opo_batch_submit(opo_batch *b, const opo_grid *g, const opo_params *p, const
         return fail(OPO_ERR_UNSUPPORTED, "grid %dx%dx%d is not supported in a batch", g->nx, g->ny, g->nt);
     if (p->dk_phase_scale != 0.0 && p->dk != 0.0)
         return fail(OPO_ERR_UNSUPPORTED, "a batch runs solves without the dk phase term");
-    if (p->no_diffraction)
-        return fail(OPO_ERR_UNSUPPORTED, "a batch runs solves with diffraction (no plane-wave model)");
+    if (g->nx == 1 && !p->no_diffraction)
+        return fail(OPO_ERR_UNSUPPORTED, "a 1x1 grid runs in a batch as the plane-wave model (no_diffraction)");
     const size_t fp = opo_batch_footprint(g, p->precision, n_trips);
     if (fp > b->budget) return fail(OPO_ERR_BUDGET, "solve needs %zu bytes, budget is %zu", fp, b->budget);
     std::lock_guard<std::mutex> lk(b->mu);
-    if (b->n_slices >= 0 && p->n_slices != b->n_slices)
-        return fail(OPO_ERR_ARG, "all solves of a batch share n_slices (%d)", b->n_slices);
-    b->n_slices = p->n_slices;
     auto e = std::make_unique<BatchEntry>();
     e->ticket = b->next_ticket++;
     e->g = *g;
@@ -1936,12 +2001,10 @@ int opo_batch_profile(opo_batch *b, int on) {
         b->profiling = want;
         b->dirty = true;
         b->prof.clear();
+        b->prof_alg.clear();
         b->prof_error.clear();
     }
-    for (opo_stage_time &e : b->prof) {   // start the sums again
-        e.launches = 0;
-        e.ms = 0.0;
-    }
+    b->prof_reset();   // start the sums again
     return OPO_OK;
 }
 
@@ -1950,7 +2013,10 @@ int opo_batch_profile_read(opo_batch *b, opo_stage_time *out, int cap, int *n) {
     std::lock_guard<std::mutex> lk(b->mu);
     if (!b->prof_error.empty()) return fail(OPO_ERR_HIP, "%s", b->prof_error.c_str());
     *n = (int)b->prof.size();
-    for (int k = 0; k < cap && k < *n; ++k) out[k] = b->prof[k];
+    for (int k = 0; k < cap && k < *n; ++k) {
+        out[k] = b->prof[k];
+        if (out[k].launches) out[k].alg_bytes = b->prof_alg[k] / out[k].launches;
+    }
     return OPO_OK;
 }
 

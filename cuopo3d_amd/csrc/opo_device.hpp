@@ -323,11 +323,7 @@ __device__ __forceinline__ void phase1_regs(cplx<T> *v, int n2, const cplx<T> *_
     }
 }
 
-// P1 / P1P false: the caller does phase 1 (phase 1') itself, with
-// phase1_regs (an inverse N1-point DFT) on registers, and the LDS holds the
-// phase-1 output on entry (the phase-1' input on return); the operations of
-// every element are the same.
-template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false, bool P1 = true, bool P1P = true>
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false>
 __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
                                             const cplx<T> *__restrict__ tw, int tid, int nthr) {
     constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
@@ -352,7 +348,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
         __syncthreads();
     } else {
         // phase 1: task (l, n2, f)
-        if constexpr (P1) {
+        {
             for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
                 const int l = task % nlines, r = uni(task / nlines), n2 = r % N2, f = r / N2;
                 cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
@@ -391,7 +387,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
         }
         __syncthreads();
         // phase 1': task (l, n2, f)
-        if constexpr (P1P) {
+        {
             for (int task = tid; task < nlines * N2 * ls.nf; task += nthr) {
                 const int l = task % nlines, r = uni(task / nlines), n2 = r % N2, f = r / N2;
                 cplx<T> *base = ls.base(f) + l * LS + n2 * ES;
@@ -444,14 +440,15 @@ __device__ __forceinline__ void phaseA_regs(cplx<T> *v, int m, const cplx<T> *__
     }
 }
 
-// PA / PAP false: the caller does phase A (phase A') itself, as with
-// line_op_set's P1 / P1P.
-template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool PA = true, bool PAP = true>
+// PG: elements between two pads of PAD (the innermost group N3 by default).
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, int PG = 0>
 __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
                                              const cplx<T> *__restrict__ tw, int tid, int nthr) {
     using Q = Split3<N>;
     constexpr int N1 = Q::N1, N2 = Q::N2, N3 = Q::N3, M = Q::M;
-    auto pos = [](int n) { return n * ES + (n / N3) * PAD; };
+    constexpr int G = PG > 0 ? PG : N3;   // elements between two pads
+    static_assert(N % G == 0, "the pad group divides the line");
+    auto pos = [](int n) { return n * ES + (n / G) * PAD; };
     auto fwd = [&](cplx<T> a, int j) {
         return S < 0 ? cmul_hl(a, tw[2 * j], tw[2 * j + 1]) : cmulc_hl(a, tw[2 * j], tw[2 * j + 1]);
     };
@@ -459,7 +456,7 @@ __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
         return S < 0 ? cmulc_hl(a, tw[2 * j], tw[2 * j + 1]) : cmul_hl(a, tw[2 * j], tw[2 * j + 1]);
     };
     // A: task (l, m, f)
-    if constexpr (PA) {
+    {
         for (int task = tid; task < nlines * M * ls.nf; task += nthr) {
             const int l = task % nlines, r = task / nlines, m = r % M, f = r / M;
             cplx<T> *base = ls.base(f) + l * LS;
@@ -525,7 +522,7 @@ __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
     }
     __syncthreads();
     // A': task (l, m, f)
-    if constexpr (PAP) {
+    {
         for (int task = tid; task < nlines * M * ls.nf; task += nthr) {
             const int l = task % nlines, r = task / nlines, m = r % M, f = r / M;
             cplx<T> *base = ls.base(f) + l * LS;
@@ -563,27 +560,13 @@ template <typename T, int N> constexpr int inner_len() {
     else return Split<N>::N2;
 }
 
-template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false, bool OUTER = true>
+// PG: pad group of a three-level line (line_op_set3); two-level lines pad
+// every N2 elements.
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false, int PG = 0>
 __device__ __forceinline__ void line_op_any(const LineSet<T> &ls, int nlines, const cplx<T> *__restrict__ tw,
                                             int tid, int nthr) {
-    if constexpr (three_level<T, N>()) line_op_set3<T, N, S, LS, ES, PAD, OUTER, OUTER>(ls, nlines, tw, tid, nthr);
-    else line_op_set<T, N, S, LS, ES, PAD, UNI, OUTER, OUTER>(ls, nlines, tw, tid, nthr);
-}
-
-// The outer level of a line operator (phase 1 / 1' of the two-level split,
-// A / A' of the three-level one): element n = (N / L) j + q of a line goes
-// through the L-point register DFT of sub-index q, L = outer_len.
-template <typename T, int N> __host__ __device__ constexpr int outer_len() {
-    if constexpr (three_level<T, N>()) return Split3<N>::N1;
-    else return Split<N>::N1;
-}
-template <typename T, int N, int S>
-__device__ __forceinline__ void outer_forward(cplx<T> *v, int q, const cplx<T> *__restrict__ tw) {
-    if constexpr (three_level<T, N>()) phaseA_regs<T, N, S>(v, q, tw);
-    else phase1_regs<T, N, S, false>(v, q, tw);
-}
-template <typename T, int N, int S> __device__ __forceinline__ void outer_inverse(cplx<T> *v) {
-    dft<outer_len<T, N>(), -S>(v);
+    if constexpr (three_level<T, N>()) line_op_set3<T, N, S, LS, ES, PAD, PG>(ls, nlines, tw, tid, nthr);
+    else line_op_set<T, N, S, LS, ES, PAD, UNI>(ls, nlines, tw, tid, nthr);
 }
 
 // One field.

@@ -163,7 +163,9 @@ enum { BATCH_FIRST = 0, BATCH_MID = 1, BATCH_LAST = 2 };
 // from Api, one half step), BATCH_MID (two half steps), BATCH_LAST (one half
 // step, mirror, GDD of s and i when the solve has it on).  variant: the
 // launch variant the work list was built for (its tile width); the
-// process-wide choice may have changed since.
+// process-wide choice may have changed since.  kBatchPlaneVariant: the
+// plane-wave solves of the class (NX = NY = 1), one pixel per workgroup.
+constexpr int kBatchPlaneVariant = -1;
 template <typename T>
 hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc<T> *sd,
                              const int *status, const WorkItem *work, int mode, int variant);

@@ -33,8 +33,12 @@ template <typename T, int NT> constexpr int tline3_tx() {
 // the line operator (N2 of the two-level split, N3 of the three-level one).
 // Narrow tiles pad every G rows by TX elements so the rows that the
 // innermost phase walks in parallel fall in different banks.
+// Rows between two pads of a narrow tile: the innermost group of the line
+// operator.  (Padding 2-pixel 2048-sample tiles every 16 rows instead,
+// conflict-free in a 64-bank model of the LDS, measured no faster.)
+template <typename T, int NT, int TX> constexpr int pad_group() { return inner_len<T, NT>(); }
 template <typename T, int NT, int TX> struct TimeTile {
-    static constexpr int G = inner_len<T, NT>();
+    static constexpr int G = pad_group<T, NT, TX>();
     static constexpr int PAD = (G > 1 && TX < 16) ? TX : 0;
     static constexpr int FT = NT * TX + (NT / G) * PAD;   // elements per field
     __device__ static __forceinline__ int at(int i) { return i + (i / (G * TX)) * PAD; }
@@ -233,7 +237,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     }
     __syncthreads();
     const LineSet<T> ls{lds + a.first_field * FT, FT, a.tab0, a.tab_stride, 3 - a.first_field};
-    line_op_any<T, NT, +1, 1, TX, L::PAD>(ls, TX, STW ? twl : tw, tid, BS);
+    line_op_any<T, NT, +1, 1, TX, L::PAD, false, L::G>(ls, TX, STW ? twl : tw, tid, BS);
     if constexpr (TX % 2 == 0) {
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const int t = i / TX, xl = i % TX;
@@ -332,7 +336,7 @@ __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T
         __syncthreads();
         if (u + 1 < u1) load(u + 1);
         const LineSet<T> ls{lds + a.first_field * FT, FT, stab + 2 * NT, 2 * NT, 3 - a.first_field};
-        line_op_any<T, NT, +1, 1, TX, L::PAD>(ls, TX, stab, tid, BS);
+        line_op_any<T, NT, +1, 1, TX, L::PAD, false, L::G>(ls, TX, stab, tid, BS);
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const size_t gi = offset(u, i);
             const int li = L::at(i);

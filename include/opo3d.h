@@ -268,7 +268,14 @@ int opo_tune_set(int precision, int axis, int n, int variant);
 int opo_batch_create(int device, size_t hbm_budget_bytes, int precision,
                      opo_batch **out);
 void opo_batch_destroy(opo_batch *b);
-/* Queue a solve for n_trips round trips; fields are host buffers (copied). */
+/* Queue a solve for n_trips round trips; fields are host buffers (copied).
+ * Solves of one batch may differ in grid, slice count (crystal length, NZ)
+ * and cavity parameters; they share the batch's precision.  Grids:
+ * NX = NY in {16, 32, 64, 96, 128 (FP32)}, or 1 x 1 with no_diffraction
+ * (the plane-wave model of Solver1D.cuh, modulator included); NT a power
+ * of two from 16 to 2048.  Solves with the dk phase term
+ * (dk_phase_scale != 0) are refused: the reference multiplies it by 0
+ * (Solver3D.cuh:375-377). */
 int opo_batch_submit(opo_batch *b, const opo_grid *grid,
                      const opo_params *params, const void *api,
                      const void *as, const void *ai, int n_trips,
@@ -329,7 +336,8 @@ typedef struct {
     int stage;          /* OPO_STAGE_TIME_OP (first and middle slices),
                            OPO_STAGE_GDD (trip-end pass), OPO_STAGE_XY_OP,
                            OPO_STAGE_FREE_SPACE or OPO_STAGE_POWER          */
-    int size;           /* NT of a t-pass class, NX of an xy class, 0 power */
+    int size;           /* NT of a t-pass class (-NT: its plane-wave solves,
+                           NX = NY = 1), NX of an xy class, 0 power        */
     int launches;       /* launches timed                                   */
     double ms;          /* their summed duration                            */
     double alg_bytes;   /* algorithmic bytes of one launch: 16 B (FP64 32 B)

@@ -71,10 +71,11 @@ template <int N, int LEVELS> constexpr int inner_group() {
     else return Split<N>::N2;
 }
 
-// Time-tile layout: element n of line (pixel) l at n*TX + l + (n / G)*PAD.
-template <int N, int TX, int S, int LEVELS> void time_tile(const char *label) {
+// Time-tile layout: element n of line (pixel) l at n*TX + l + (n / G)*PAD,
+// G the innermost group of the operator or the pad group PG.
+template <int N, int TX, int S, int LEVELS, int PG = 0> void time_tile(const char *label) {
     constexpr int NF = 3, PAD = TX < 16 ? TX : 0;
-    constexpr int G = inner_group<N, LEVELS>();
+    constexpr int G = PG > 0 ? PG : inner_group<N, LEVELS>();
     constexpr int FT = N * TX + (N / G) * PAD;
     Case c = make<N, S>(TX, NF, 1000 * N + TX);
     std::vector<cplx<float>> lds((size_t)NF * FT);
@@ -86,7 +87,7 @@ template <int N, int TX, int S, int LEVELS> void time_tile(const char *label) {
                 lds[(size_t)f * FT + at(n, l)] = {(float)v.real(), (float)v.imag()};
             }
     const LineSet<float> ls{lds.data(), FT, c.tabf.data(), 2 * N, NF};
-    if constexpr (LEVELS == 3) line_op_set3<float, N, S, 1, TX, PAD>(ls, TX, c.tw.data(), 0, 1);
+    if constexpr (LEVELS == 3) line_op_set3<float, N, S, 1, TX, PAD, PG>(ls, TX, c.tw.data(), 0, 1);
     else line_op_set<float, N, S, 1, TX, PAD>(ls, TX, c.tw.data(), 0, 1);
     std::vector<cd> out(c.x.size());
     for (int f = 0; f < NF; ++f)
@@ -131,6 +132,7 @@ int main() {
     time_tile<1024, 2, -1, 3>("t1024_three_level");
     time_tile<2048, 2, +1, 3>("t2048_three_level");
     time_tile<2048, 1, -1, 3>("t2048_three_level_one_pixel");
+    time_tile<2048, 2, -1, 3, 16>("t2048_three_level_pad16");
     xy_plane<32, true, false>("x32_rows");
     xy_plane<64, true, true>("x64_rows_uniform");
     xy_plane<64, false, true>("x64_columns_uniform");

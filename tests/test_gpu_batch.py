@@ -206,10 +206,9 @@ def test_batch_with_eom_equals_single_solves(require_gpu):
         assert np.array_equal(ps, pb)   # the same reduction order alone and batched
 
 
-def test_batch_rejects_plane_wave_solves():
-    from cuopo3d_amd.capi import Grid
-    g, p, ph, f = configs.cavity_64(n=(32, 32, 64), n_round_trips=1)
-    p.no_diffraction = 1
+def test_batch_rejects_1x1_grid_with_diffraction():
+    """A 1 x 1 grid runs in a batch only as the plane-wave model."""
+    g, p, ph, f = configs.cavity_64(n=(1, 1, 256), n_round_trips=1)
     b = co.Batch(1 << 30)
     with pytest.raises(RuntimeError, match="plane-wave"):
         b.submit(g, p, *f, 1)
@@ -313,3 +312,79 @@ def test_batch_concurrent_classes_bitwise(require_gpu):
         for a, b in zip(fa, fg):
             assert np.array_equal(a, b), k
         assert np.array_equal(pw, pg), k
+
+
+def test_batch_mixed_slice_counts_bitwise(require_gpu):
+    """Solves of different crystal lengths and slice counts (the float slice
+    rule of Solver3D.cuh:949) in one batch: a size class's launch at slice k
+    covers only its solves with more than k slices, so each solve's fields
+    and per-trip powers are bitwise those of the solve alone through the
+    single-solve API, also after the shortest solve has left the batch."""
+    from cuopo3d_amd.capi import Grid
+    draws = configs.sweep_draws(n_solves=4, seed=11)
+    specs = [(32, 128, 5000.0, 100, 3), (32, 128, 3000.0, 31, 2), (64, 64, 2000.0, 41, 3), (32, 256, 5000.0, 60, 3)]
+    solves = []
+    for d, (nxy, nt, lcr, nz, trips) in zip(draws, specs):
+        grid = Grid(nxy, nxy, nt)
+        ph = configs.physical(lcr=lcr, nz=nz, rs=d["R"], phase_s=d["phase"], n_round_trips=trips)
+        p = configs.make_params(grid, ph)
+        f = configs.make_fields(grid, p, ph, power=d["power"], waist=d["waist"], seed=d["seed"])
+        solves.append((grid, p, ph, f, trips))
+    slices = [s[1].n_slices for s in solves]
+    print("slices per solve", slices, flush=True)
+    assert len(set(slices)) == 4
+    b = co.Batch(8 * GB)
+    tickets = [b.submit(g, p, *f, trips) for (g, p, ph, f, trips) in solves]
+    results = {}
+    for _ in range(10):
+        b.step()
+        for t in tickets:
+            if t not in results:
+                r = b.poll(t)
+                if r is not None:
+                    results[t] = r
+        if len(results) == len(tickets):
+            break
+    b.close()
+    for k, (g, p, ph, f, trips) in enumerate(solves):
+        with co.Solver(g, p, device=0) as s:
+            s.set_fields(*f)
+            ps = s.runOPO(trips)
+            fs = s.get_fields()
+        pb, fb = results[tickets[k]]
+        for a, c in zip(fs, fb):
+            assert np.array_equal(a, c), k
+        assert np.array_equal(ps, pb), k
+
+
+def test_batch_plane_wave_solves_bitwise(require_gpu):
+    """Plane-wave solves (NX = NY = 1, no transverse step, with and without
+    the modulator and GDD; Solver1D.cuh:436-445, 62-72) batched beside 3-D
+    solves of the same NT: their t pass is a size class of its own (one
+    pixel per workgroup) and they skip the xy pass and free space, so each
+    solve's fields and per-trip powers are bitwise those of the solve alone
+    through the single-solve API."""
+    solves = []
+    for nt, gdd, beta in ((256, 1, 0.5), (256, 0, 0.3), (1024, 1, 0.0)):
+        grid, p, ph, f = configs.cavity_64(n=(1, 1, nt), n_round_trips=3)
+        p.no_diffraction = 1
+        p.gdd_on = gdd
+        if beta:
+            p.eom_on, p.eom_beta, p.eom_fpm = 1, beta, 2.0 / p.t_window
+        solves.append((grid, p, ph, f))
+    # the plane-wave model on a 3-D grid: the usual t pass, no xy pass
+    g, p, ph, f = configs.cavity_64(n=(32, 32, 64), n_round_trips=3)
+    p.no_diffraction = 1
+    solves.append((g, p, ph, f))
+    solves += mix(4, trips=3)   # 3-D solves, one of them 32 x 32 x 256
+    got, _ = run_and_collect(solves, 3)
+    for k, (g, p, ph, f) in enumerate(solves):
+        with co.Solver(g, p, device=0) as s:
+            s.set_fields(*f)
+            ps = s.runOPO(3)
+            fs = s.get_fields()
+        pb, fb = got[k]
+        for a, c in zip(fs, fb):
+            assert np.array_equal(a, c), k
+        assert np.array_equal(ps, pb), k
+    print("plane-wave and 3-D solves batched == alone, bitwise", flush=True)
