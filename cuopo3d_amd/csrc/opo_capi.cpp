@@ -142,7 +142,7 @@ template <typename T> struct SolveT final : SolveImpl {
     using C = cplx<T>;
     C *Api = nullptr, *A[3] = {nullptr, nullptr, nullptr};
     C *tables = nullptr;
-    C *tw_x = nullptr, *tw_t = nullptr, *tabT[3] = {}, *tabG = nullptr, *qx[3] = {}, *qy[3] = {},
+    C *tw_x = nullptr, *tw_y = nullptr, *tw_t = nullptr, *tabT[3] = {}, *tabG = nullptr, *qx[3] = {}, *qy[3] = {},
       *qfsx[2] = {}, *qfsy[2] = {}, *tabE = nullptr;
     double *dbuf = nullptr;   // partials | pw3
     double *partial = nullptr, *pw3 = nullptr;
@@ -199,16 +199,16 @@ template <typename T> struct SolveT final : SolveImpl {
         return OPO_OK;
     }
 
-    // Entries (cplx<T>) of the table block: twiddles for NX and NT, the
+    // Entries (cplx<T>) of the table block: twiddles for NX, NY and NT, the
     // three D tables, GDD, the x and y factors of Q for three fields, the
     // free-space x and y factors for s and i, and the modulator table, each
     // entry a (hi, lo) pair.
     static size_t table_entries(const opo_grid &g) {
-        return 2 * ((size_t)g.nx + g.nt + 3 * (size_t)g.nt + g.nt + 3 * (size_t)g.nx + 3 * (size_t)g.ny +
+        return 2 * ((size_t)g.nx + g.ny + g.nt + 3 * (size_t)g.nt + g.nt + 3 * (size_t)g.nx + 3 * (size_t)g.ny +
                     2 * (size_t)g.nx + 2 * (size_t)g.ny + g.nt);
     }
     struct TableOffsets {
-        size_t twx, twt, D, G, qx, qy, fx, fy, E;
+        size_t twx, twy, twt, D, G, qx, qy, fx, fy, E;
     };
     static TableOffsets table_offsets(const opo_grid &g) {
         const int nx = g.nx, ny = g.ny, nt = g.nt;
@@ -216,6 +216,7 @@ template <typename T> struct SolveT final : SolveImpl {
         auto take = [&](size_t n) { size_t o = off; off += 2 * n; return o; };
         TableOffsets o;
         o.twx = take(nx);
+        o.twy = take(ny);
         o.twt = take(nt);
         o.D = take(3 * (size_t)nt);
         o.G = take(nt);
@@ -248,6 +249,11 @@ template <typename T> struct SolveT final : SolveImpl {
             double c, s;
             unit_root_neg(j, nx, c, s);
             put(o_twx + 2 * (size_t)j, c, s);
+        }
+        for (int j = 0; j < ny; ++j) {
+            double c, s;
+            unit_root_neg(j, ny, c, s);
+            put(o.twy + 2 * (size_t)j, c, s);
         }
         for (int j = 0; j < nt; ++j) {
             double c, s;
@@ -319,6 +325,7 @@ template <typename T> struct SolveT final : SolveImpl {
         const TableOffsets o = table_offsets(g);
         tables = block;
         tw_x = tables + o.twx;
+        tw_y = tables + o.twy;
         tw_t = tables + o.twt;
         for (int f = 0; f < 3; ++f) {
             tabT[f] = tables + o.D + 2 * (size_t)f * g.nt;
@@ -454,13 +461,13 @@ template <typename T> struct SolveT final : SolveImpl {
     int q_diffraction(const int *st) {
         FieldOps<T> ops{};
         for (int f = 0; f < 3; ++f) { ops.f[f] = A[f]; ops.tab[f] = qx[f]; ops.taby[f] = qy[f]; }
-        PROF(OPO_STAGE_XY_OP, (launch_xyplane<T>(stream, geo(), ops, 3, tw_x, st)));
+        PROF(OPO_STAGE_XY_OP, (launch_xyplane<T>(stream, geo(), ops, 3, tw_x, tw_y, st)));
         return OPO_OK;
     }
     int q_free_space(const int *st) {
         FieldOps<T> ops{};
         for (int f = 0; f < 2; ++f) { ops.f[f] = A[1 + f]; ops.tab[f] = qfsx[f]; ops.taby[f] = qfsy[f]; }
-        PROF(OPO_STAGE_FREE_SPACE, (launch_xyplane<T>(stream, geo(), ops, 2, tw_x, st)));
+        PROF(OPO_STAGE_FREE_SPACE, (launch_xyplane<T>(stream, geo(), ops, 2, tw_x, tw_y, st)));
         return OPO_OK;
     }
 
@@ -784,6 +791,7 @@ template <typename T> struct SolveT final : SolveImpl {
         const int order[3] = {0, 1, 0};
         for (int round = 0; round < 3; ++round) {
             const int axis = order[round];
+            if (axis == 1 && g.nx != g.ny) continue;   // NX != NY planes have one launch shape
             const int n = axis == 0 ? g.nt : g.nx;
             int best = tune_get(fp64, axis, n);
             double best_t = 1e30;

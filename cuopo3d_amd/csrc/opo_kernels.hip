@@ -481,18 +481,71 @@ static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<
     }
 }
 
-bool xyplane_supported(int nx, int ny, bool fp64) {
-    if (nx != ny) return false;
-    switch (nx) {
-        case 1: case 16: case 32: case 64: case 96: case 128: return true;
+static bool line_len_supported(int n) {
+    switch (n) {
+        case 16: case 32: case 64: case 96: case 128: return true;
         default: return false;
+    }
+}
+
+bool xyplane_supported(int nx, int ny, bool fp64) {
+    if (nx != ny) return line_len_supported(nx) && line_len_supported(ny);
+    return nx == 1 || line_len_supported(nx);
+}
+
+// NX != NY: Q_x on bands of rows, then Q_y on bands of columns, each band
+// in LDS (k_xrows, k_ycols); bands of 32 rows or columns where the other
+// dimension allows, else 16 (every supported length is a multiple of 16).
+template <typename T, int NX>
+static hipError_t rows_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
+                              const cplx<T> *tw, const int *status) {
+    if (g.ny % 32 == 0)
+        hipLaunchKernelGGL((k_xrows<T, NX, 32, 256>), dim3((unsigned)(g.nt * (g.ny / 32)), (unsigned)nf), dim3(256),
+                           0, st, ops, g.ny, tw, status);
+    else
+        hipLaunchKernelGGL((k_xrows<T, NX, 16, 256>), dim3((unsigned)(g.nt * (g.ny / 16)), (unsigned)nf), dim3(256),
+                           0, st, ops, g.ny, tw, status);
+    return hipGetLastError();
+}
+template <typename T, int NY>
+static hipError_t cols_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
+                              const cplx<T> *tw, const int *status) {
+    if (g.nx % 32 == 0)
+        hipLaunchKernelGGL((k_ycols<T, NY, 32, 256>), dim3((unsigned)(g.nt * (g.nx / 32)), (unsigned)nf), dim3(256),
+                           0, st, ops, g.nx, tw, status);
+    else
+        hipLaunchKernelGGL((k_ycols<T, NY, 16, 256>), dim3((unsigned)(g.nt * (g.nx / 16)), (unsigned)nf), dim3(256),
+                           0, st, ops, g.nx, tw, status);
+    return hipGetLastError();
+}
+template <typename T>
+static hipError_t xy_rect_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
+                                 const cplx<T> *tw_x, const cplx<T> *tw_y, const int *status) {
+    if (!line_len_supported(g.nx) || !line_len_supported(g.ny) || g.nx % 16 || g.ny % 16)
+        return hipErrorInvalidValue;
+    hipError_t e = hipErrorInvalidValue;
+    switch (g.nx) {
+        case 16: e = rows_launch<T, 16>(st, g, ops, nf, tw_x, status); break;
+        case 32: e = rows_launch<T, 32>(st, g, ops, nf, tw_x, status); break;
+        case 64: e = rows_launch<T, 64>(st, g, ops, nf, tw_x, status); break;
+        case 96: e = rows_launch<T, 96>(st, g, ops, nf, tw_x, status); break;
+        case 128: e = rows_launch<T, 128>(st, g, ops, nf, tw_x, status); break;
+    }
+    if (e != hipSuccess) return e;
+    switch (g.ny) {
+        case 16: return cols_launch<T, 16>(st, g, ops, nf, tw_y, status);
+        case 32: return cols_launch<T, 32>(st, g, ops, nf, tw_y, status);
+        case 64: return cols_launch<T, 64>(st, g, ops, nf, tw_y, status);
+        case 96: return cols_launch<T, 96>(st, g, ops, nf, tw_y, status);
+        case 128: return cols_launch<T, 128>(st, g, ops, nf, tw_y, status);
+        default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
 hipError_t launch_xyplane(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
-                          const cplx<T> *tw, const int *status) {
-    if (g.nx != g.ny) return hipErrorInvalidValue;
+                          const cplx<T> *tw, const cplx<T> *tw_y, const int *status) {
+    if (g.nx != g.ny) return xy_rect_launch<T>(st, g, ops, nf, tw, tw_y, status);
     switch (g.nx) {
         case 1: return xy_dispatch<T, 1>(st, g, ops, nf, tw, status);
         case 16: return xy_dispatch<T, 16>(st, g, ops, nf, tw, status);
@@ -591,7 +644,7 @@ hipError_t launch_power_pixel(hipStream_t st, const cplx<T> *f, size_t plane, in
     template hipError_t launch_tline3<T>(hipStream_t, const Geometry &, const TLine3Args<T> &,      \
                                          const cplx<T> *, const int *);                           \
     template hipError_t launch_xyplane<T>(hipStream_t, const Geometry &, const FieldOps<T> &, int, \
-                                          const cplx<T> *, const int *);                          \
+                                          const cplx<T> *, const cplx<T> *, const int *);         \
     template hipError_t launch_power_partial<T>(hipStream_t, const cplx<T> *, const cplx<T> *,     \
                                                 const cplx<T> *, size_t, int, double *, const int *); \
     template hipError_t launch_power_pixel<T>(hipStream_t, const cplx<T> *, size_t, int, double,   \

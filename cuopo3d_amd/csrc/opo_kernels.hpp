@@ -66,10 +66,13 @@ template <typename T>
 hipError_t launch_tline(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                         const cplx<T> *tw_t, const int *status);
 
-// xy operator Q_y Q_x (each F^{+}( tab . F^{-}(A) )) on every t-plane of nf fields.
+// xy operator Q_y Q_x (each F^{+}( tab . F^{-}(A) )) on every t-plane of nf
+// fields; tw_x, tw_y the NX- and NY-point twiddles (the same table when
+// NX = NY).  Square planes run as one pass through LDS, NX != NY planes as
+// a pass over bands of rows then one over bands of columns.
 template <typename T>
 hipError_t launch_xyplane(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
-                          const cplx<T> *tw_x, const int *status);
+                          const cplx<T> *tw_x, const cplx<T> *tw_y, const int *status);
 
 // Power partial sums: partial[f*nblk + b], double, Neumaier-compensated per
 // thread and reduced in a fixed tree order per block.

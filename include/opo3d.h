@@ -46,10 +46,13 @@ extern "C" {
 #define OPO_FP32 0   /* FP32 bulk; FP64-built phases reduced mod 2pi        */
 #define OPO_FP64 1   /* all-FP64 device path                                 */
 
-/* Supported axis lengths of the device kernels.  nx, ny in {1, 16, 32, 64,
- * 96, 128}; nt in {1, 16, 32, 64, 128, 256, 512, 1024, 2048}.  A length of 1 turns
- * the axis off (NX = NY = 1 is the reference's 1D plane-wave build,
- * src/headers/Solver1D.cuh; NT = 1 its CW build, DtypesConsts.cuh:38). */
+/* Supported axis lengths of the device kernels.  nx, ny in {16, 32, 64,
+ * 96, 128}, independently (the reference declares NX and NY separately,
+ * DtypesConsts.cuh:28-31; NX != NY planes take two passes through HBM for
+ * the xy operator instead of one), or nx = ny = 1; nt in {1, 16, 32, 64,
+ * 128, 256, 512, 1024, 2048}.  A length of 1 turns the axis off (NX = NY =
+ * 1 is the reference's 1D plane-wave build, src/headers/Solver1D.cuh;
+ * NT = 1 its CW build, DtypesConsts.cuh:38).  Batches take NX = NY. */
 
 typedef struct {
     int nx, ny, nt;

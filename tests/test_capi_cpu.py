@@ -43,7 +43,11 @@ def test_create_rejects_bad_input():
     assert lib.opo_create(C.byref(Grid(0, 32, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_ARG
     assert lib.opo_create(C.byref(Grid(100, 100, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_UNSUPPORTED
     assert b"not supported" in lib.opo_last_error()
-    assert lib.opo_create(C.byref(Grid(32, 64, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_UNSUPPORTED
+    # NX != NY: both lengths must be supported line lengths
+    assert lib.opo_create(C.byref(Grid(32, 48, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_UNSUPPORTED
+    assert lib.opo_create(C.byref(Grid(1, 32, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_UNSUPPORTED
+    if lib.opo_device_count() == 0:   # a supported rectangular grid gets past validation
+        assert lib.opo_create(C.byref(Grid(32, 64, 64)), C.byref(p), 0, C.byref(h)) == OPO_ERR_NOGPU
     bad = p.copy()
     bad.precision = 7
     assert lib.opo_create(C.byref(g), C.byref(bad), 0, C.byref(h)) == OPO_ERR_ARG
@@ -76,7 +80,7 @@ def test_batch_footprint():
     g = Grid(128, 128, 1024)
     fp = lib.opo_batch_footprint(C.byref(g), 0, 300)
     assert 4 * 8 * 128 * 128 * 1024 <= fp < 4 * 8 * 128 * 128 * 1024 * 1.01
-    # tables (6 NX + 5 NY + 6 NT entries, hi/lo pairs), partials, counters and
+    # tables (6 NX + 6 NY + 6 NT entries, hi/lo pairs), partials, counters and
     # the 3-double power series of every trip each take at least one 64 KiB unit
     small = Grid(16, 16, 16)
     f1 = lib.opo_batch_footprint(C.byref(small), 0, 1)

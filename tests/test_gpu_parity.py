@@ -61,7 +61,46 @@ def test_diffraction_operator(n, require_gpu):
     _check(s, o, 1e-6, f"Q {n}")
 
 
-@pytest.mark.parametrize("n", [(32, 32, 64), (128, 128, 16)], ids=lambda n: "x".join(map(str, n)))
+# NX != NY: Q_x on bands of rows and Q_y on bands of columns (two passes)
+RECT_GRIDS = [(64, 32, 16), (32, 96, 16), (128, 16, 16), (16, 128, 32)]
+
+
+@pytest.mark.parametrize("n", RECT_GRIDS, ids=lambda n: "x".join(map(str, n)))
+def test_diffraction_operator_rectangular(n, require_gpu):
+    grid, p, _ = cavity_params(n)
+    s, o = _pair(grid, p, random_fields(grid, 14))
+    s.diffractionInCrystal()
+    o.diffraction()
+    _check(s, o, 1e-6, f"Q {n}")
+
+
+@pytest.mark.parametrize("n", [(128, 64, 16), (32, 128, 16)], ids=lambda n: "x".join(map(str, n)))
+def test_diffraction_operator_rectangular_fp64(n, require_gpu):
+    grid, p, _ = cavity_params(n, precision=OPO_FP64)
+    s, o = _pair(grid, p, random_fields(grid, 15, precision=OPO_FP64))
+    s.diffractionInCrystal()
+    o.diffraction()
+    _check(s, o, 1e-12, f"Q fp64 {n}")
+
+
+@pytest.mark.parametrize("n,prec", [((64, 32, 128), OPO_FP32), ((32, 96, 64), OPO_FP64)],
+                         ids=["64x32x128", "32x96x64-fp64"])
+def test_rectangular_cavity_against_oracle(n, prec, require_gpu):
+    """A cavity on an NX != NY grid (the reference declares NX and NY
+    separately, DtypesConsts.cuh:28-31): 2 trips against the oracle, fields
+    to 1e-5 (FP32) or 1e-11 (FP64) and the per-trip signal power to 1e-6 or
+    1e-11."""
+    grid, p, ph, fields = configs.cavity_64(n=n, n_round_trips=2, precision=prec)
+    s, o = _pair(grid, p, fields)
+    pg = s.runOPO(2)
+    pr = o.run_round_trips(2)
+    tol = 1e-5 if prec == OPO_FP32 else 1e-11
+    _check(s, o, tol, f"rectangular cavity {n}")
+    rel = np.abs(pg - pr) / np.abs(pr)
+    assert rel[:, 1].max() < (1e-6 if prec == OPO_FP32 else 1e-11), rel
+
+
+@pytest.mark.parametrize("n", [(32, 32, 64), (128, 128, 16), (64, 32, 64)], ids=lambda n: "x".join(map(str, n)))
 def test_free_space_and_gdd_and_mirror(n, require_gpu):
     grid, p, _ = cavity_params(n, rs=0.9, phase_s=0.7)
     s, o = _pair(grid, p, random_fields(grid, 13))
