@@ -72,6 +72,7 @@ def main():
                          "many round trips of the 64x64x256 cavity (checkpoints); off by default")
     ap.add_argument("--cavity-grid", type=int, nargs=2, metavar=("NXY", "NT"),
                     help="instead of the sweep, time one cavity solve on this grid (profiling)")
+    ap.add_argument("--fp64", action="store_true", help="with --cavity-grid: the all-FP64 device path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 128 MiB device-to-device copy first (counter calibration)")
     args = ap.parse_args()
@@ -469,22 +470,30 @@ def one_at_a_time_leg(co, configs, batched, trips=3):
 
 def reference_grid_leg(co, configs, tune_reps, trips=3):
     """The cavity on the grid the reference compiles by default
-    (NX = NY = 128, NT = 2048: DtypesConsts.cuh:30-54), tuned for it."""
-    g, p, _, f = configs.cavity_64(n=(128, 128, 2048), n_round_trips=trips + 1)
-    s = co.Solver(g, p, device=0)
-    s.set_fields(*f)
-    report = s.autotune(reps=max(1, min(tune_reps, 2)))
-    s.runOPO(1)
-    s.sync()
-    t0 = time.perf_counter()
-    s.runOPO(trips)
-    s.sync()
-    dt = time.perf_counter() - t0
-    s.close()
-    cells = g.nx * g.ny * g.nt
-    return {"grid": [g.nx, g.ny, g.nt], "trips": trips, "value": cells * p.n_slices * trips / dt,
-            "unit": "cell-steps/s", "ms_per_step": dt / trips * 1e3,
-            "autotune": [ln[-12:] for ln in report.strip().splitlines()]}
+    (NX = NY = 128, NT = 2048: DtypesConsts.cuh:30-54), tuned for it, on the
+    FP32 path and on the FP64 path (the mode that holds 1e-5 over long runs)."""
+    from cuopo3d_amd.capi import OPO_FP64
+    out = {}
+    for name, prec in (("fp32", 0), ("fp64", OPO_FP64)):
+        g, p, _, f = configs.cavity_64(n=(128, 128, 2048), n_round_trips=trips + 1, precision=prec)
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        report = s.autotune(reps=max(1, min(tune_reps, 2)))
+        s.runOPO(1)
+        s.sync()
+        t0 = time.perf_counter()
+        s.runOPO(trips)
+        s.sync()
+        dt = time.perf_counter() - t0
+        s.close()
+        cells = g.nx * g.ny * g.nt
+        out[name] = {"value": cells * p.n_slices * trips / dt, "unit": "cell-steps/s",
+                     "ms_per_step": dt / trips * 1e3,
+                     "autotune": [ln[-12:] for ln in report.strip().splitlines()]}
+        log(f"128x128x2048 {name}: {dt / trips * 1e3:.1f} ms per trip")
+    out.update({"grid": [128, 128, 2048], "trips": trips, "value": out["fp32"]["value"],
+                "unit": "cell-steps/s", "ms_per_step": out["fp32"]["ms_per_step"]})
+    return out
 
 
 def plain_leg(one_at_a_time, batched, solves=8, trips=1):
@@ -571,16 +580,17 @@ def cavity_profile_mode(co, configs, lib, args):
     run without graph capture (opo_profile_trips)."""
     from cuopo3d_amd.capi import STAGES
     n = (args.cavity_grid[0], args.cavity_grid[0], args.cavity_grid[1])
-    grid, p, ph, fields = configs.cavity_64(n=n, n_round_trips=args.warmup + args.steps + 1)
+    prec = 1 if args.fp64 else 0
+    grid, p, ph, fields = configs.cavity_64(n=n, n_round_trips=args.warmup + args.steps + 1, precision=prec)
     cells = grid.nx * grid.ny * grid.nt
     s = co.Solver(grid, p, device=0)
     s.set_fields(*fields)
     if args.variants_in:
-        variants_set(lib, load_variants(args.variants_in))
+        variants_set(lib, load_variants(args.variants_in), prec)
     elif not args.no_tune:
         s.autotune(reps=args.tune_reps)
     if args.variants_out:
-        json.dump(variants_get(lib), open(args.variants_out, "w"))
+        json.dump(variants_get(lib, prec), open(args.variants_out, "w"))
     s.runOPO(args.warmup)
     s.sync()
     t0 = time.perf_counter()
@@ -594,12 +604,13 @@ def cavity_profile_mode(co, configs, lib, args):
     for k, name in enumerate(STAGES):
         if nl[k]:
             avg = ms[k] / nl[k]
-            alg = cells * STAGE_FIELDS[name] * 8 * STAGE_RW.get(name, 2)
+            alg = cells * STAGE_FIELDS[name] * (16 if prec else 8) * STAGE_RW.get(name, 2)
             stages[name] = {"launches": nl[k], "avg_us": avg * 1e3, "alg_bytes": alg,
                             "GBs": alg / (avg * 1e-3) / 1e9}
-    return {"workload": f"cavity {grid.nx}x{grid.ny}x{grid.nt}", "value": cells * p.n_slices * args.steps / dt,
+    return {"workload": f"cavity {grid.nx}x{grid.ny}x{grid.nt}" + (" fp64" if prec else ""),
+            "value": cells * p.n_slices * args.steps / dt,
             "unit": "cell-steps/s", "ms_per_trip": dt / args.steps * 1e3, "stages_unbatched_events": stages,
-            "variants": variants_get(lib)}
+            "variants": variants_get(lib, prec)}
 
 
 if __name__ == "__main__":

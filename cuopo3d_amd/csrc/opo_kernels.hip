@@ -173,6 +173,8 @@ unsigned persistent_grid(int units, int per_cu) {
 }
 
 // ----------------------------------------------------- variant tables
+constexpr int kSplitVariants = 3;   // xy variants of the two-pass form (xy_split_launch)
+
 // Chosen variant per (precision, size): process-wide, read at every launch
 // and written by the tuner, so the entries are atomics (a launch sees either
 // the old or the new choice, both of which give the same results).
@@ -248,7 +250,8 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
         return fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
     }
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
-    if (n < 16 || (fp64 && n == 128)) return v == 0;   // FP64 128: the two-pass form only
+    if (n < 16) return v == 0;
+    if (fp64 && n == 128) return v < kSplitVariants;   // FP64 128: the two-pass form
     if (n <= 32) return v < 6;
     if (xy_variant_persistent(n, v)) return !fp64 && n * n / xy_variant_bs(n, v) <= 16;
     return true;
@@ -292,7 +295,7 @@ int tune_per_cu(bool fp64, int axis, int n, int v) {
 }
 
 int tune_block(bool fp64, int axis, int n, int v) {
-    if (axis == 1) return xy_variant_bs(n, v);
+    if (axis == 1) return (fp64 && n == 128) ? (v == 1 ? 512 : 256) : xy_variant_bs(n, v);
     int tx = 0, bs = 0;
     const bool ok = fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
     return ok ? bs : 0;
@@ -425,17 +428,27 @@ __global__ void __launch_bounds__(BS) k_ycols(FieldOps<T> ops, int nx, const cpl
 // workgroups per CU).
 constexpr int kSplitBand = 32;
 
-template <typename T, int NX>
-static hipError_t xy_split_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
-                                  const cplx<T> *tw, const int *status) {
-    if (g.ny != NX || NX % kSplitBand != 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_xrows<T, NX, kSplitBand, 256>), dim3((unsigned)(g.nt * (NX / kSplitBand)), (unsigned)nf),
-                       dim3(256), 0, st, ops, g.ny, tw, status);
+template <typename T, int NX, int B, int BS>
+static hipError_t xy_split_launch_b(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
+                                    const cplx<T> *tw, const int *status) {
+    if (g.ny != NX || NX % B != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_xrows<T, NX, B, BS>), dim3((unsigned)(g.nt * (NX / B)), (unsigned)nf), dim3(BS), 0, st,
+                       ops, g.ny, tw, status);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ycols<T, NX, kSplitBand, 256>), dim3((unsigned)(g.nt * (NX / kSplitBand)), (unsigned)nf),
-                       dim3(256), 0, st, ops, g.nx, tw, status);
+    hipLaunchKernelGGL((k_ycols<T, NX, B, BS>), dim3((unsigned)(g.nt * (NX / B)), (unsigned)nf), dim3(BS), 0, st,
+                       ops, g.nx, tw, status);
     return hipGetLastError();
+}
+// Variants of the two-pass form (FP64 128 x 128 planes): v = 0 bands of 32
+// with 256 threads, 1 bands of 32 with 512, 2 bands of 16 with 256 (four
+// workgroups per CU); kSplitVariants of them.
+template <typename T, int NX>
+static hipError_t xy_split_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
+                                  const cplx<T> *tw, const int *status, int v = 0) {
+    if (v == 1) return xy_split_launch_b<T, NX, kSplitBand, 512>(st, g, ops, nf, tw, status);
+    if (v == 2) return xy_split_launch_b<T, NX, 16, 256>(st, g, ops, nf, tw, status);
+    return xy_split_launch_b<T, NX, kSplitBand, 256>(st, g, ops, nf, tw, status);
 }
 
 template <typename T, int NX, int BS, int P>
@@ -554,7 +567,7 @@ hipError_t launch_xyplane(hipStream_t st, const Geometry &g, const FieldOps<T> &
         case 96: return xy_dispatch<T, 96>(st, g, ops, nf, tw, status);
         case 128:
             if constexpr (sizeof(T) == 4) return xy_dispatch<T, 128>(st, g, ops, nf, tw, status);
-            else return xy_split_launch<T, 128>(st, g, ops, nf, tw, status);
+            else return xy_split_launch<T, 128>(st, g, ops, nf, tw, status, tune_get(true, 1, 128));
         default: return hipErrorInvalidValue;
     }
 }

@@ -191,10 +191,13 @@ def test_batch_follows_tuning_changes_between_trips(restore_tuning, require_gpu)
         assert np.array_equal(pg, pr)
 
 
-@pytest.mark.parametrize("n", [(32, 32, 64), (64, 64, 256)], ids=lambda n: "x".join(map(str, n)))
+@pytest.mark.parametrize("n", [(32, 32, 64), (64, 64, 256), (16, 16, 2048), (128, 128, 64)],
+                         ids=lambda n: "x".join(map(str, n)))
 def test_fp64_variants_give_the_same_fields(n, require_gpu):
-    """The FP64 path's launch variants (one tile or plane per workgroup, or
-    groups of small planes) give bitwise the fields of variant 0."""
+    """The FP64 path's launch variants (one tile or plane per workgroup,
+    groups of small planes, one-pixel 2048-sample tiles on 256, 512 or 384
+    threads, the band shapes of the two-pass 128 x 128 xy operator) give
+    bitwise the fields of variant 0."""
     from cuopo3d_amd.capi import OPO_FP64
     lib = co.load()
     nx, ny, nt = n
