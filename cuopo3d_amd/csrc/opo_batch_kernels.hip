@@ -164,6 +164,54 @@ __global__ void __launch_bounds__(BS) kb_xy(const SolveDesc<T> *__restrict__ sd,
     xy_plane<T, NX, BS, P>(A, tx, ty, d.twx, lds);
 }
 
+// FP64 128 x 128 planes (256 KiB, beyond the LDS): Q_x on bands of RB rows,
+// then (a second launch over the same work list) Q_y on bands of RB
+// columns, each workgroup walking the bands of its plane; every line gets
+// the operations of the single-solve row and column passes, so the fields
+// are bitwise those of the solve alone.
+template <typename T, int N, int RB, int BS, bool ROWS>
+__global__ void __launch_bounds__(BS) kb_xyband(const SolveDesc<T> *__restrict__ sd,
+                                                const TripState *__restrict__ ts,
+                                                const int *__restrict__ status,
+                                                const WorkItem *__restrict__ work, int free_space) {
+    constexpr int BP = RB + 1;
+    __shared__ cplx<T> lds[ROWS ? RB * (N + 1) : N * BP];
+    const WorkItem w = work[xcd_local_tile(blockIdx.x, gridDim.x)];
+    if (status[w.solve]) return;
+    if (free_space && !ts[w.solve].fs) return;
+    const SolveDesc<T> &d = sd[w.solve];
+    const int k = w.item / d.nt, t = w.item % d.nt;
+    const int f = free_space ? 1 + k : k;
+    const cplx<T> *tab = ROWS ? (free_space ? d.qfsx : d.qx) + 2 * N * k : (free_space ? d.qfsy : d.qy) + 2 * N * k;
+    cplx<T> *A = (f == 0 ? d.f[0] : (f == 1 ? d.f[1] : d.f[2])) + (size_t)t * N * N;
+    for (int b0 = 0; b0 < N; b0 += RB) {
+        if constexpr (ROWS) {   // rows b0 .. b0 + RB - 1, pitch N + 1
+            for (int i = threadIdx.x; i < RB * N; i += BS) lds[(i / N) * (N + 1) + i % N] = A[(size_t)b0 * N + i];
+            __syncthreads();
+            line_op<T, N, -1, N + 1, 1>(lds, RB, d.twx, tab, threadIdx.x, BS);
+            for (int i = threadIdx.x; i < RB * N; i += BS) A[(size_t)b0 * N + i] = lds[(i / N) * (N + 1) + i % N];
+        } else {                // columns b0 .. b0 + RB - 1, each line down a column
+            for (int i = threadIdx.x; i < N * RB; i += BS) lds[(i / RB) * BP + i % RB] = A[(size_t)(i / RB) * N + b0 + i % RB];
+            __syncthreads();
+            line_op<T, N, -1, 1, BP>(lds, RB, d.twx, tab, threadIdx.x, BS);
+            for (int i = threadIdx.x; i < N * RB; i += BS) A[(size_t)(i / RB) * N + b0 + i % RB] = lds[(i / RB) * BP + i % RB];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int RB, int BS>
+static hipError_t bx_band_launch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
+                                 const int *status, const WorkItem *work, int fs) {
+    hipLaunchKernelGGL((kb_xyband<T, 128, RB, BS, true>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts, status,
+                       work, fs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((kb_xyband<T, 128, RB, BS, false>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts, status,
+                       work, fs);
+    return hipGetLastError();
+}
+
 // Power of every resident solve, in the order a solve uses alone
 // (opo_power.hpp): work item (solve, b) is partial block b of the solve's
 // power_blocks(cells) blocks, then one 256-thread workgroup per solve.
@@ -204,7 +252,7 @@ bool batch_supported(int nx, int ny, int nt, bool fp64) {
     if (nx != ny || !tline_supported(nt, fp64) || nt < 16) return false;
     switch (nx) {
         case 1: case 16: case 32: case 64: case 96: return true;
-        case 128: return !fp64;
+        case 128: return true;
         default: return false;
     }
 }
@@ -296,8 +344,13 @@ hipError_t launch_batch_xy(hipStream_t st, int nx, int n_work, const SolveDesc<T
         case 64: return bx_dispatch<T, 64>(st, n_work, sd, ts, status, work, free_space, variant);
         case 96: return bx_dispatch<T, 96>(st, n_work, sd, ts, status, work, free_space, variant);
         case 128:
-            if constexpr (sizeof(T) == 4) return bx_dispatch<T, 128>(st, n_work, sd, ts, status, work, free_space, variant);
-            else return hipErrorInvalidValue;
+            if constexpr (sizeof(T) == 4) {
+                return bx_dispatch<T, 128>(st, n_work, sd, ts, status, work, free_space, variant);
+            } else {   // the band shapes of the single-solve two-pass variants
+                if (variant == 1) return bx_band_launch<T, 32, 512>(st, n_work, sd, ts, status, work, free_space);
+                if (variant == 2) return bx_band_launch<T, 16, 256>(st, n_work, sd, ts, status, work, free_space);
+                return bx_band_launch<T, 32, 256>(st, n_work, sd, ts, status, work, free_space);
+            }
         default: return hipErrorInvalidValue;
     }
 }

@@ -274,7 +274,7 @@ void opo_batch_destroy(opo_batch *b);
 /* Queue a solve for n_trips round trips; fields are host buffers (copied).
  * Solves of one batch may differ in grid, slice count (crystal length, NZ)
  * and cavity parameters; they share the batch's precision.  Grids:
- * NX = NY in {16, 32, 64, 96, 128 (FP32)}, or 1 x 1 with no_diffraction
+ * NX = NY in {16, 32, 64, 96, 128}, or 1 x 1 with no_diffraction
  * (the plane-wave model of Solver1D.cuh, modulator included); NT a power
  * of two from 16 to 2048.  Solves with the dk phase term
  * (dk_phase_scale != 0) are refused: the reference multiplies it by 0

@@ -240,10 +240,11 @@ def test_cpp_driver_runs(tmp_path, require_gpu):
 
 def test_fp64_batch_equals_single_solves(require_gpu):
     """An FP64 batch (all-FP64 device path) gives the FP64 single-solve
-    fields bitwise and the oracle's to 1e-11."""
+    fields bitwise and the oracle's to 1e-11, 128 x 128 planes (row bands
+    then column bands) included."""
     from cuopo3d_amd.capi import OPO_FP64
     solves = []
-    for n in ((32, 32, 64), (64, 64, 32)):
+    for n in ((32, 32, 64), (64, 64, 32), (128, 128, 32)):
         g, p, ph, f = configs.cavity_64(n=n, n_round_trips=2, precision=OPO_FP64)
         solves.append((g, p, ph, f))
     b = co.Batch(4 * GB, precision=OPO_FP64)
