@@ -297,6 +297,9 @@ hipError_t launch_batch_time(hipStream_t st, int nt, int n_work, const SolveDesc
         case 512: return bt_dispatch<T, 512>(st, n_work, sd, status, work, mode, variant);
         case 1024: return bt_dispatch<T, 1024>(st, n_work, sd, status, work, mode, variant);
         case 2048: return bt_dispatch<T, 2048>(st, n_work, sd, status, work, mode, variant);
+        case 4096:
+            if constexpr (sizeof(T) == 4) return bt_dispatch<T, 4096>(st, n_work, sd, status, work, mode, variant);
+            else return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }

@@ -408,9 +408,9 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
 // pixels still gives each thread a task in every phase, at the price of two
 // more LDS exchanges and one more pair of twiddle multiplies.
 template <int N> struct Split3 {
-    static constexpr int N1 = (N == 256) ? 8 : (N == 512) ? 8 : (N == 1024) ? 16 : (N == 2048) ? 16 : 0;
-    static constexpr int N2 = (N == 256) ? 8 : (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 16 : 0;
-    static constexpr int N3 = (N == 256) ? 4 : (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 8 : 0;
+    static constexpr int N1 = (N == 256) ? 8 : (N == 512) ? 8 : (N == 1024) ? 16 : (N == 2048) ? 16 : (N == 4096) ? 16 : 0;
+    static constexpr int N2 = (N == 256) ? 8 : (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 16 : (N == 4096) ? 16 : 0;
+    static constexpr int N3 = (N == 256) ? 4 : (N == 512) ? 8 : (N == 1024) ? 8 : (N == 2048) ? 8 : (N == 4096) ? 16 : 0;
     static constexpr int M = N2 * N3;
     static_assert(N1 * N2 * N3 == N, "unsupported three-level line length");
 };
@@ -551,7 +551,7 @@ __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
 //    levels 8 * 4 * 4: 3 % faster at 64 x 64, 4 % slower at 96 x 96), and
 //    so do FP32 512-sample lines (three levels within +-4 % by grid).
 template <typename T, int N> constexpr bool three_level() {
-    return N == 2048 || N == 1024 || (sizeof(T) == 8 && (N == 512 || N == 256));
+    return N >= 1024 || (sizeof(T) == 8 && (N == 512 || N == 256));
 }
 // Length of the innermost group of a line (the groups that padding
 // separates in LDS).

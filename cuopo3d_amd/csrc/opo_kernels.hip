@@ -111,9 +111,10 @@ static hipError_t tline_dispatch(hipStream_t st, const Geometry &g, const FieldO
     return hipGetLastError();
 }
 
-bool tline_supported(int nt, bool) {
+bool tline_supported(int nt, bool fp64) {
     switch (nt) {
         case 1: case 16: case 32: case 64: case 128: case 256: case 512: case 1024: case 2048: return true;
+        case 4096: return !fp64;   // three fields of a 4096-sample line fill 102 KiB of LDS in FP32
         default: return false;
     }
 }
@@ -131,6 +132,9 @@ hipError_t launch_tline(hipStream_t st, const Geometry &g, const FieldOps<T> &op
         case 512: return tline_dispatch<T, 512>(st, g, ops, nf, tw, status);
         case 1024: return tline_dispatch<T, 1024>(st, g, ops, nf, tw, status);
         case 2048: return tline_dispatch<T, 2048>(st, g, ops, nf, tw, status);
+        case 4096:
+            if constexpr (sizeof(T) == 4) return tline_dispatch<T, 4096>(st, g, ops, nf, tw, status);
+            else return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
@@ -179,12 +183,12 @@ constexpr int kSplitVariants = 3;   // xy variants of the two-pass form (xy_spli
 // and written by the tuner, so the entries are atomics (a launch sees either
 // the old or the new choice, both of which give the same results).
 struct TuneTables {
-    std::atomic<int> time[2][12];   // [fp64][log2 NT]
+    std::atomic<int> time[2][13];   // [fp64][log2 NT]
     std::atomic<int> xy[2][8];      // [fp64][NX index]
     TuneTables() {
         const int nxs[6] = {1, 16, 32, 64, 96, 128};
         for (int f = 0; f < 2; ++f) {
-            for (int k = 0; k < 12; ++k) time[f][k].store(time_default_variant(f != 0, 1 << k));
+            for (int k = 0; k < 13; ++k) time[f][k].store(time_default_variant(f != 0, 1 << k));
             for (int k = 0; k < 8; ++k) xy[f][k].store(k < 6 ? xy_default_variant(nxs[k]) : 0);
         }
     }
@@ -227,6 +231,9 @@ static bool time_info(int nt, int v, int &tx, int &bs, int &per_cu) {
     switch (nt) {
 #define TI(N) case N: time_variant_info<T, N>(v, valid, tx, bs, per_cu); break;
         TI(1) TI(16) TI(32) TI(64) TI(128) TI(256) TI(512) TI(1024) TI(2048)
+        case 4096:
+            if constexpr (sizeof(T) == 4) time_variant_info<T, 4096>(v, valid, tx, bs, per_cu);
+            break;
 #undef TI
         default: return false;
     }
@@ -261,7 +268,7 @@ int tune_get(bool fp64, int axis, int n) {
     TuneTables &tt = tune_tables();
     if (axis == 0) {
         const int l = log2i(n);
-        return (l >= 0 && l < 12) ? tt.time[fp64][l].load(std::memory_order_relaxed) : 0;
+        return (l >= 0 && l < 13) ? tt.time[fp64][l].load(std::memory_order_relaxed) : 0;
     }
     const int k = nx_index(n);
     return k >= 0 ? tt.xy[fp64][k].load(std::memory_order_relaxed) : 0;
@@ -364,6 +371,9 @@ hipError_t launch_tline3(hipStream_t st, const Geometry &g, const TLine3Args<T> 
         case 512: return tline3_dispatch<T, 512>(st, g, a, tw, status);
         case 1024: return tline3_dispatch<T, 1024>(st, g, a, tw, status);
         case 2048: return tline3_dispatch<T, 2048>(st, g, a, tw, status);
+        case 4096:
+            if constexpr (sizeof(T) == 4) return tline3_dispatch<T, 4096>(st, g, a, tw, status);
+            else return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }

@@ -82,7 +82,7 @@ template <typename T, int NT, int V> struct TimeVariant {
     static constexpr int bs = persistent ? (b == 0 ? 256 : (b == 1 ? 384 : 512)) : (b == 0 ? 256 : (b == 1 ? 512 : 384));
     // one-pixel tiles only where nothing wider fits (FP64 2048-sample lines:
     // there they give the 384- and 512-thread blocks)
-    static constexpr int min_tx = (sizeof(T) == 8 && NT >= 2048) ? 1 : 2;
+    static constexpr int min_tx = ((sizeof(T) == 8 && NT >= 2048) || NT >= 4096) ? 1 : 2;
     static constexpr int data_bytes = 3 * TimeTile<T, NT, (tx > 0 ? tx : 1)>::FT * (int)sizeof(cplx<T>);
     static constexpr int table_bytes = 8 * NT * (int)sizeof(cplx<T>);  // twiddles + 3 tables, hi/lo
     static constexpr bool valid =

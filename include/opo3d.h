@@ -50,7 +50,9 @@ extern "C" {
  * 96, 128}, independently (the reference declares NX and NY separately,
  * DtypesConsts.cuh:28-31; NX != NY planes take two passes through HBM for
  * the xy operator instead of one), or nx = ny = 1; nt in {1, 16, 32, 64,
- * 128, 256, 512, 1024, 2048}.  A length of 1 turns the axis off (NX = NY =
+ * 128, 256, 512, 1024, 2048}, and 4096 on the FP32 path (three fields of
+ * one pixel's 4096-sample line fill 102 KiB of LDS; FP64 would need twice
+ * that).  A length of 1 turns the axis off (NX = NY =
  * 1 is the reference's 1D plane-wave build, src/headers/Solver1D.cuh;
  * NT = 1 its CW build, DtypesConsts.cuh:38).  Batches take NX = NY. */
 
@@ -276,7 +278,7 @@ void opo_batch_destroy(opo_batch *b);
  * and cavity parameters; they share the batch's precision.  Grids:
  * NX = NY in {16, 32, 64, 96, 128}, or 1 x 1 with no_diffraction
  * (the plane-wave model of Solver1D.cuh, modulator included); NT a power
- * of two from 16 to 2048.  Solves with the dk phase term
+ * of two from 16 to 2048 (4096 in FP32).  Solves with the dk phase term
  * (dk_phase_scale != 0) are refused: the reference multiplies it by 0
  * (Solver3D.cuh:375-377). */
 int opo_batch_submit(opo_batch *b, const opo_grid *grid,

@@ -132,6 +132,7 @@ int main() {
     time_tile<1024, 2, -1, 3>("t1024_three_level");
     time_tile<2048, 2, +1, 3>("t2048_three_level");
     time_tile<2048, 1, -1, 3>("t2048_three_level_one_pixel");
+    time_tile<4096, 1, +1, 3>("t4096_three_level_one_pixel");
     time_tile<2048, 2, -1, 3, 16>("t2048_three_level_pad16");
     xy_plane<32, true, false>("x32_rows");
     xy_plane<64, true, true>("x64_rows_uniform");

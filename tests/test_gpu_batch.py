@@ -63,15 +63,15 @@ def test_batch_equals_single_solve_and_oracle(require_gpu):
 
 
 def test_batch_long_and_grouped_lines_equal_single_solves(require_gpu):
-    """A 2048-sample solve (three-level t operator) and 16- and 32-pixel
-    solves (xy groups of planes when tuned so) batched together give the
+    """2048- and 4096-sample solves (three-level t operators) and 16- and
+    32-pixel solves (xy groups of planes when tuned so) batched together give the
     single-solve fields bitwise, with the small-plane xy variant set to a
     group of planes and to one plane."""
     from cuopo3d_amd.capi import OPO_FP32, OPO_TUNE_XY_PASS, OPO_OK
     lib = co.load()
     draws = configs.sweep_draws(n_solves=4, seed=5)
     solves = []
-    for d, (nxy, nt) in zip(draws, [(16, 2048), (32, 128), (16, 256)]):
+    for d, (nxy, nt) in zip(draws, [(16, 2048), (32, 128), (16, 256), (16, 4096)]):
         d = dict(d)
         d["nxy"], d["nt"] = nxy, nt
         solves.append(configs.sweep_solve(d, n_round_trips=2))

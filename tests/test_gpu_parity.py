@@ -18,7 +18,7 @@ from helpers import rel_l2, random_fields, cavity_params
 pytestmark = pytest.mark.gpu
 
 OP_GRIDS = [(16, 16, 16), (32, 32, 64), (64, 64, 32), (96, 96, 16), (128, 128, 16),
-            (16, 16, 128), (16, 16, 256), (16, 16, 512), (16, 16, 1024), (16, 16, 2048),
+            (16, 16, 128), (16, 16, 256), (16, 16, 512), (16, 16, 1024), (16, 16, 2048), (16, 16, 4096),
             (1, 1, 64)]
 
 
@@ -59,6 +59,25 @@ def test_diffraction_operator(n, require_gpu):
     s.diffractionInCrystal()
     o.diffraction()
     _check(s, o, 1e-6, f"Q {n}")
+
+
+@pytest.mark.parametrize("n", [(16, 16, 4096), (1, 1, 4096)], ids=lambda n: "x".join(map(str, n)))
+def test_4096_sample_lines_against_oracle(n, require_gpu):
+    """4096-sample time lines (FP32; one pixel's three fields fill 102 KiB
+    of LDS, the three-level 16 x 16 x 16 operator): two cavity trips of a
+    16 x 16 grid and of the plane-wave model against the oracle, fields to
+    1e-5 and per-trip signal power to 1e-6 (3-D) or the stated plane-wave
+    level."""
+    grid, p, ph, fields = configs.cavity_64(n=n, n_round_trips=2)
+    if n[0] == 1:
+        p.no_diffraction = 1
+    s, o = _pair(grid, p, fields)
+    pg = s.runOPO(2)
+    pr = o.run_round_trips(2)
+    _check(s, o, 1e-5, f"4096-sample lines {n}")
+    rel = np.abs(pg - pr) / np.abs(pr)
+    print("per-trip power rel err (p, s, i)", rel.max(axis=0), flush=True)
+    assert rel[:, 1].max() < (1e-6 if n[0] > 1 else PLANE_WAVE_FP32_SIGNAL_POWER), rel
 
 
 # NX != NY: Q_x on bands of rows and Q_y on bands of columns (two passes)

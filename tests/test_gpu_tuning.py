@@ -19,7 +19,7 @@ from helpers import rel_l2, batch_mix, batch_run
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(64, 64, 256), (32, 32, 1024), (128, 128, 64), (16, 16, 512), (16, 16, 2048)]
+GRIDS = [(64, 64, 256), (32, 32, 1024), (128, 128, 64), (16, 16, 512), (16, 16, 2048), (16, 16, 4096)]
 
 
 @pytest.fixture

@@ -28,7 +28,7 @@ def results(tmp_path_factory):
 
 
 CASES = ["t256_two_level", "t512_two_level", "t1024_two_level", "t256_three_level", "t512_three_level", "t1024_three_level",
-         "t2048_three_level", "t2048_three_level_one_pixel",
+         "t2048_three_level", "t2048_three_level_one_pixel", "t4096_three_level_one_pixel",
          "t2048_three_level_pad16", "x32_rows", "x64_rows_uniform",
          "x64_columns_uniform", "x96_columns", "x128_rows_uniform"]
 
