@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(BS) kb_xy(const SolveDesc<T> *__restrict__ sd,
     xy_plane<T, NX, BS, P>(A, tx, ty, d.twx, lds);
 }
 
-// FP64 128 x 128 planes (256 KiB, beyond the LDS): Q_x on bands of RB rows,
+// FP64 128 x 128 and all 256 x 256 planes (beyond the LDS): Q_x on bands of RB rows,
 // then (a second launch over the same work list) Q_y on bands of RB
 // columns, each workgroup walking the bands of its plane; every line gets
 // the operations of the single-solve row and column passes, so the fields
@@ -200,16 +200,26 @@ __global__ void __launch_bounds__(BS) kb_xyband(const SolveDesc<T> *__restrict__
     }
 }
 
-template <typename T, int RB, int BS>
+template <typename T, int N, int RB, int BS>
 static hipError_t bx_band_launch(hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
                                  const int *status, const WorkItem *work, int fs) {
-    hipLaunchKernelGGL((kb_xyband<T, 128, RB, BS, true>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts, status,
+    hipLaunchKernelGGL((kb_xyband<T, N, RB, BS, true>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts, status,
                        work, fs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((kb_xyband<T, 128, RB, BS, false>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts, status,
+    hipLaunchKernelGGL((kb_xyband<T, N, RB, BS, false>), dim3((unsigned)n_work), dim3(BS), 0, st, sd, ts, status,
                        work, fs);
     return hipGetLastError();
+}
+// The band shapes of the single-solve two-pass variants (FP64 128 x 128 and
+// all 256 x 256 planes): 1 bands of 32 on 512 threads, 2 bands of 16 on
+// 256, else bands of 32 on 256.
+template <typename T, int N>
+static hipError_t bx_band_variant(hipStream_t st, int n_work, const SolveDesc<T> *sd, const TripState *ts,
+                                  const int *status, const WorkItem *work, int fs, int variant) {
+    if (variant == 1) return bx_band_launch<T, N, 32, 512>(st, n_work, sd, ts, status, work, fs);
+    if (variant == 2) return bx_band_launch<T, N, 16, 256>(st, n_work, sd, ts, status, work, fs);
+    return bx_band_launch<T, N, 32, 256>(st, n_work, sd, ts, status, work, fs);
 }
 
 // Power of every resident solve, in the order a solve uses alone
@@ -251,8 +261,7 @@ __global__ void __launch_bounds__(256) kb_power_final(const SolveDesc<T> *__rest
 bool batch_supported(int nx, int ny, int nt, bool fp64) {
     if (nx != ny || !tline_supported(nt, fp64) || nt < 16) return false;
     switch (nx) {
-        case 1: case 16: case 32: case 64: case 96: return true;
-        case 128: return true;
+        case 1: case 16: case 32: case 64: case 96: case 128: case 256: return true;
         default: return false;
     }
 }
@@ -347,13 +356,9 @@ hipError_t launch_batch_xy(hipStream_t st, int nx, int n_work, const SolveDesc<T
         case 64: return bx_dispatch<T, 64>(st, n_work, sd, ts, status, work, free_space, variant);
         case 96: return bx_dispatch<T, 96>(st, n_work, sd, ts, status, work, free_space, variant);
         case 128:
-            if constexpr (sizeof(T) == 4) {
-                return bx_dispatch<T, 128>(st, n_work, sd, ts, status, work, free_space, variant);
-            } else {   // the band shapes of the single-solve two-pass variants
-                if (variant == 1) return bx_band_launch<T, 32, 512>(st, n_work, sd, ts, status, work, free_space);
-                if (variant == 2) return bx_band_launch<T, 16, 256>(st, n_work, sd, ts, status, work, free_space);
-                return bx_band_launch<T, 32, 256>(st, n_work, sd, ts, status, work, free_space);
-            }
+            if constexpr (sizeof(T) == 4) return bx_dispatch<T, 128>(st, n_work, sd, ts, status, work, free_space, variant);
+            else return bx_band_variant<T, 128>(st, n_work, sd, ts, status, work, free_space, variant);
+        case 256: return bx_band_variant<T, 256>(st, n_work, sd, ts, status, work, free_space, variant);
         default: return hipErrorInvalidValue;
     }
 }

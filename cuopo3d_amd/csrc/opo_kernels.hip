@@ -186,10 +186,10 @@ struct TuneTables {
     std::atomic<int> time[2][13];   // [fp64][log2 NT]
     std::atomic<int> xy[2][8];      // [fp64][NX index]
     TuneTables() {
-        const int nxs[6] = {1, 16, 32, 64, 96, 128};
+        const int nxs[7] = {1, 16, 32, 64, 96, 128, 256};
         for (int f = 0; f < 2; ++f) {
             for (int k = 0; k < 13; ++k) time[f][k].store(time_default_variant(f != 0, 1 << k));
-            for (int k = 0; k < 8; ++k) xy[f][k].store(k < 6 ? xy_default_variant(nxs[k]) : 0);
+            for (int k = 0; k < 8; ++k) xy[f][k].store(k < 7 ? xy_default_variant(nxs[k]) : 0);
         }
     }
 };
@@ -206,7 +206,7 @@ static int log2i(int n) {
 static int nx_index(int nx) {
     switch (nx) {
         case 1: return 0; case 16: return 1; case 32: return 2; case 64: return 3;
-        case 96: return 4; case 128: return 5; default: return -1;
+        case 96: return 4; case 128: return 5; case 256: return 6; default: return -1;
     }
 }
 
@@ -258,7 +258,7 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
     }
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
     if (n < 16) return v == 0;
-    if (fp64 && n == 128) return v < kSplitVariants;   // FP64 128: the two-pass form
+    if (n == 256 || (fp64 && n == 128)) return v < kSplitVariants;   // the two-pass form
     if (n <= 32) return v < 6;
     if (xy_variant_persistent(n, v)) return !fp64 && n * n / xy_variant_bs(n, v) <= 16;
     return true;
@@ -302,7 +302,7 @@ int tune_per_cu(bool fp64, int axis, int n, int v) {
 }
 
 int tune_block(bool fp64, int axis, int n, int v) {
-    if (axis == 1) return (fp64 && n == 128) ? (v == 1 ? 512 : 256) : xy_variant_bs(n, v);
+    if (axis == 1) return (n == 256 || (fp64 && n == 128)) ? (v == 1 ? 512 : 256) : xy_variant_bs(n, v);
     int tx = 0, bs = 0;
     const bool ok = fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
     return ok ? bs : 0;
@@ -450,7 +450,7 @@ static hipError_t xy_split_launch_b(hipStream_t st, const Geometry &g, const Fie
                        ops, g.nx, tw, status);
     return hipGetLastError();
 }
-// Variants of the two-pass form (FP64 128 x 128 planes): v = 0 bands of 32
+// Variants of the two-pass form (FP64 128 x 128 and all 256 x 256 planes): v = 0 bands of 32
 // with 256 threads, 1 bands of 32 with 512, 2 bands of 16 with 256 (four
 // workgroups per CU); kSplitVariants of them.
 template <typename T, int NX>
@@ -506,7 +506,7 @@ static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<
 
 static bool line_len_supported(int n) {
     switch (n) {
-        case 16: case 32: case 64: case 96: case 128: return true;
+        case 16: case 32: case 64: case 96: case 128: case 256: return true;
         default: return false;
     }
 }
@@ -519,11 +519,18 @@ bool xyplane_supported(int nx, int ny, bool fp64) {
 // NX != NY: Q_x on bands of rows, then Q_y on bands of columns, each band
 // in LDS (k_xrows, k_ycols); bands of 32 rows or columns where the other
 // dimension allows, else 16 (every supported length is a multiple of 16).
+// 256-point lines take the fastest square-plane band shapes: FP32 bands of
+// 32 on 512 threads (one 16-point DFT per thread and phase), FP64 bands of
+// 16 (64 KiB of LDS, two workgroups per CU).
+template <typename T, int N> constexpr int max_band() { return (sizeof(T) == 8 && N == 256) ? 16 : 32; }
+template <typename T, int N> constexpr int band32_threads() { return (sizeof(T) == 4 && N == 256) ? 512 : 256; }
+
 template <typename T, int NX>
 static hipError_t rows_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                               const cplx<T> *tw, const int *status) {
-    if (g.ny % 32 == 0)
-        hipLaunchKernelGGL((k_xrows<T, NX, 32, 256>), dim3((unsigned)(g.nt * (g.ny / 32)), (unsigned)nf), dim3(256),
+    constexpr int B32 = band32_threads<T, NX>();
+    if (max_band<T, NX>() == 32 && g.ny % 32 == 0)
+        hipLaunchKernelGGL((k_xrows<T, NX, 32, B32>), dim3((unsigned)(g.nt * (g.ny / 32)), (unsigned)nf), dim3(B32),
                            0, st, ops, g.ny, tw, status);
     else
         hipLaunchKernelGGL((k_xrows<T, NX, 16, 256>), dim3((unsigned)(g.nt * (g.ny / 16)), (unsigned)nf), dim3(256),
@@ -533,8 +540,9 @@ static hipError_t rows_launch(hipStream_t st, const Geometry &g, const FieldOps<
 template <typename T, int NY>
 static hipError_t cols_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                               const cplx<T> *tw, const int *status) {
-    if (g.nx % 32 == 0)
-        hipLaunchKernelGGL((k_ycols<T, NY, 32, 256>), dim3((unsigned)(g.nt * (g.nx / 32)), (unsigned)nf), dim3(256),
+    constexpr int B32 = band32_threads<T, NY>();
+    if (max_band<T, NY>() == 32 && g.nx % 32 == 0)
+        hipLaunchKernelGGL((k_ycols<T, NY, 32, B32>), dim3((unsigned)(g.nt * (g.nx / 32)), (unsigned)nf), dim3(B32),
                            0, st, ops, g.nx, tw, status);
     else
         hipLaunchKernelGGL((k_ycols<T, NY, 16, 256>), dim3((unsigned)(g.nt * (g.nx / 16)), (unsigned)nf), dim3(256),
@@ -553,6 +561,7 @@ static hipError_t xy_rect_launch(hipStream_t st, const Geometry &g, const FieldO
         case 64: e = rows_launch<T, 64>(st, g, ops, nf, tw_x, status); break;
         case 96: e = rows_launch<T, 96>(st, g, ops, nf, tw_x, status); break;
         case 128: e = rows_launch<T, 128>(st, g, ops, nf, tw_x, status); break;
+        case 256: e = rows_launch<T, 256>(st, g, ops, nf, tw_x, status); break;
     }
     if (e != hipSuccess) return e;
     switch (g.ny) {
@@ -561,6 +570,7 @@ static hipError_t xy_rect_launch(hipStream_t st, const Geometry &g, const FieldO
         case 64: return cols_launch<T, 64>(st, g, ops, nf, tw_y, status);
         case 96: return cols_launch<T, 96>(st, g, ops, nf, tw_y, status);
         case 128: return cols_launch<T, 128>(st, g, ops, nf, tw_y, status);
+        case 256: return cols_launch<T, 256>(st, g, ops, nf, tw_y, status);
         default: return hipErrorInvalidValue;
     }
 }
@@ -578,6 +588,7 @@ hipError_t launch_xyplane(hipStream_t st, const Geometry &g, const FieldOps<T> &
         case 128:
             if constexpr (sizeof(T) == 4) return xy_dispatch<T, 128>(st, g, ops, nf, tw, status);
             else return xy_split_launch<T, 128>(st, g, ops, nf, tw, status, tune_get(true, 1, 128));
+        case 256: return xy_split_launch<T, 256>(st, g, ops, nf, tw, status, tune_get(sizeof(T) == 8, 1, 256));
         default: return hipErrorInvalidValue;
     }
 }

@@ -275,6 +275,40 @@ def test_fp64_batch_equals_single_solves(require_gpu):
         assert max(errs) < 1e-11
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_batch_256_planes_equal_single_solves(prec, require_gpu):
+    """A 256 x 256 solve (row-band then column-band xy passes) batched with
+    a 32 x 32 one gives the single-solve fields and per-trip power bitwise,
+    in either precision."""
+    from cuopo3d_amd.capi import OPO_FP64
+    pr = OPO_FP64 if prec == "fp64" else OPO_FP32
+    solves = [configs.cavity_64(n=n, n_round_trips=2, precision=pr) for n in ((256, 256, 16), (32, 32, 64))]
+    b = co.Batch(4 * GB, precision=pr)
+    tickets = [b.submit(g, p, *f, 2) for (g, p, ph, f) in solves]
+    res = {}
+    for _ in range(20):
+        b.step()
+        for t in tickets:
+            if t not in res:
+                r = b.poll(t)
+                if r is not None:
+                    res[t] = r
+        if len(res) == len(tickets):
+            break
+    b.close()
+    assert len(res) == len(tickets)
+    for (g, p, ph, f), t in zip(solves, tickets):
+        pb, fb = res[t]
+        s = co.Solver(g, p, device=0)
+        s.set_fields(*f)
+        ps = s.runOPO(2)
+        for a, c in zip(s.get_fields(), fb):
+            assert np.array_equal(a, c), (prec, g.nx)
+        assert np.array_equal(ps, pb), (prec, g.nx)
+        s.close()
+    print(prec, "256 x 256 batch == single solve", flush=True)
+
+
 def test_batch_concurrent_classes_bitwise(require_gpu):
     """Size classes of a stage run as parallel branches of the trip graph
     (opo_batch_concurrent_classes): the fields and per-trip powers of every

@@ -17,7 +17,7 @@ from helpers import rel_l2, random_fields, cavity_params
 
 pytestmark = pytest.mark.gpu
 
-OP_GRIDS = [(16, 16, 16), (32, 32, 64), (64, 64, 32), (96, 96, 16), (128, 128, 16),
+OP_GRIDS = [(16, 16, 16), (32, 32, 64), (64, 64, 32), (96, 96, 16), (128, 128, 16), (256, 256, 16),
             (16, 16, 128), (16, 16, 256), (16, 16, 512), (16, 16, 1024), (16, 16, 2048), (16, 16, 4096),
             (1, 1, 64)]
 
@@ -81,7 +81,7 @@ def test_4096_sample_lines_against_oracle(n, require_gpu):
 
 
 # NX != NY: Q_x on bands of rows and Q_y on bands of columns (two passes)
-RECT_GRIDS = [(64, 32, 16), (32, 96, 16), (128, 16, 16), (16, 128, 32)]
+RECT_GRIDS = [(64, 32, 16), (32, 96, 16), (128, 16, 16), (16, 128, 32), (256, 32, 16), (64, 256, 16)]
 
 
 @pytest.mark.parametrize("n", RECT_GRIDS, ids=lambda n: "x".join(map(str, n)))
@@ -93,7 +93,8 @@ def test_diffraction_operator_rectangular(n, require_gpu):
     _check(s, o, 1e-6, f"Q {n}")
 
 
-@pytest.mark.parametrize("n", [(128, 64, 16), (32, 128, 16)], ids=lambda n: "x".join(map(str, n)))
+@pytest.mark.parametrize("n", [(128, 64, 16), (32, 128, 16), (256, 256, 16), (256, 64, 16)],
+                         ids=lambda n: "x".join(map(str, n)))
 def test_diffraction_operator_rectangular_fp64(n, require_gpu):
     grid, p, _ = cavity_params(n, precision=OPO_FP64)
     s, o = _pair(grid, p, random_fields(grid, 15, precision=OPO_FP64))
@@ -209,6 +210,24 @@ def test_fp64_128_planes(require_gpu):
     pr = o.run_round_trips(2)
     _check(s, o, 1e-11, "fp64 2 trips 128x128x64")
     assert (np.abs(pg - pr) / np.abs(pr)).max() < 1e-11
+
+
+@pytest.mark.parametrize("n,prec", [((256, 256, 32), OPO_FP32), ((256, 256, 16), OPO_FP64)],
+                         ids=["256x256x32", "256x256x16-fp64"])
+def test_256_planes_cavity_against_oracle(n, prec, require_gpu):
+    """256 x 256 planes (0.5 MiB FP32, 1 MiB FP64 per plane, beyond the
+    LDS): the xy operator as a pass over row bands then one over column
+    bands.  Two trips (so the free-space step runs too) against the oracle:
+    fields to 1e-5 (FP32) or 1e-11 (FP64), per-trip signal power to 1e-6 or
+    1e-11."""
+    grid, p, ph, fields = configs.cavity_64(n=n, n_round_trips=2, precision=prec)
+    s, o = _pair(grid, p, fields)
+    pg = s.runOPO(2)
+    pr = o.run_round_trips(2)
+    tol = 1e-5 if prec == OPO_FP32 else 1e-11
+    _check(s, o, tol, f"256 planes {n}")
+    rel = np.abs(pg - pr) / np.abs(pr)
+    assert rel[:, 1].max() < (1e-6 if prec == OPO_FP32 else 1e-11), rel
 
 
 def test_fp64_reference_default_grid_two_slices(require_gpu):

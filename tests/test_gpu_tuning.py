@@ -19,16 +19,17 @@ from helpers import rel_l2, batch_mix, batch_run
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(64, 64, 256), (32, 32, 1024), (128, 128, 64), (16, 16, 512), (16, 16, 2048), (16, 16, 4096)]
+GRIDS = [(64, 64, 256), (32, 32, 1024), (128, 128, 64), (16, 16, 512), (16, 16, 2048), (16, 16, 4096),
+         (256, 256, 32)]
 
 
 @pytest.fixture
 def restore_tuning():
     lib = co.load()
     saved = {}
-    for nt in (16, 32, 64, 128, 256, 512, 1024, 2048):
+    for nt in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096):
         saved[(OPO_TUNE_TIME_PASS, nt)] = lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, nt)
-    for nx in (16, 32, 64, 96, 128):
+    for nx in (16, 32, 64, 96, 128, 256):
         saved[(OPO_TUNE_XY_PASS, nx)] = lib.opo_tune_get(OPO_FP32, OPO_TUNE_XY_PASS, nx)
     yield lib
     for (axis, n), v in saved.items():
@@ -191,13 +192,13 @@ def test_batch_follows_tuning_changes_between_trips(restore_tuning, require_gpu)
         assert np.array_equal(pg, pr)
 
 
-@pytest.mark.parametrize("n", [(32, 32, 64), (64, 64, 256), (16, 16, 2048), (128, 128, 64)],
+@pytest.mark.parametrize("n", [(32, 32, 64), (64, 64, 256), (16, 16, 2048), (128, 128, 64), (256, 256, 64)],
                          ids=lambda n: "x".join(map(str, n)))
 def test_fp64_variants_give_the_same_fields(n, require_gpu):
     """The FP64 path's launch variants (one tile or plane per workgroup,
     groups of small planes, one-pixel 2048-sample tiles on 256, 512 or 384
-    threads, the band shapes of the two-pass 128 x 128 xy operator) give
-    bitwise the fields of variant 0."""
+    threads, the band shapes of the two-pass 128 x 128 and 256 x 256 xy
+    operators) give bitwise the fields of variant 0."""
     from cuopo3d_amd.capi import OPO_FP64
     lib = co.load()
     nx, ny, nt = n
