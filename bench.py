@@ -39,7 +39,7 @@ STAGE_RW = {"power": 1}  # the power reduction only reads
 BATCH_KERNEL = {"time_op": "kb_time", "gdd": "kb_time (trip end)", "xy_op": "kb_xy",
                 "free_space": "kb_xy (free space)", "power": "kb_power"}
 NT_LIST = (16, 32, 64, 128, 256, 512, 1024, 2048, 4096)
-NX_LIST = (16, 32, 64, 96, 128, 256)
+NX_LIST = (16, 32, 64, 96, 128, 256, 512)
 
 
 def log(*a):

@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(BS) kb_xy(const SolveDesc<T> *__restrict__ sd,
     xy_plane<T, NX, BS, P>(A, tx, ty, d.twx, lds);
 }
 
-// FP64 128 x 128 and all 256 x 256 planes (beyond the LDS): Q_x on bands of RB rows,
+// FP64 128 x 128 and all 256 x 256 and 512 x 512 planes (beyond the LDS): Q_x on bands of RB rows,
 // then (a second launch over the same work list) Q_y on bands of RB
 // columns, each workgroup walking the bands of its plane; every line gets
 // the operations of the single-solve row and column passes, so the fields
@@ -186,15 +186,21 @@ __global__ void __launch_bounds__(BS) kb_xyband(const SolveDesc<T> *__restrict__
     cplx<T> *A = (f == 0 ? d.f[0] : (f == 1 ? d.f[1] : d.f[2])) + (size_t)t * N * N;
     for (int b0 = 0; b0 < N; b0 += RB) {
         if constexpr (ROWS) {   // rows b0 .. b0 + RB - 1, pitch N + 1
-            for (int i = threadIdx.x; i < RB * N; i += BS) lds[(i / N) * (N + 1) + i % N] = A[(size_t)b0 * N + i];
+            cplx<T> *R = A + (size_t)b0 * N;
+            chunked_copy<T, RB * N, BS>(threadIdx.x, [&](int i) { return R[i]; },
+                                        [&](int i, cplx<T> v) { lds[(i / N) * (N + 1) + i % N] = v; });
             __syncthreads();
-            line_op<T, N, -1, N + 1, 1>(lds, RB, d.twx, tab, threadIdx.x, BS);
-            for (int i = threadIdx.x; i < RB * N; i += BS) A[(size_t)b0 * N + i] = lds[(i / N) * (N + 1) + i % N];
+            band_line_op<T, N, -1, N + 1, 1>(lds, RB, d.twx, tab, threadIdx.x, BS);
+            chunked_copy<T, RB * N, BS>(threadIdx.x, [&](int i) { return lds[(i / N) * (N + 1) + i % N]; },
+                                        [&](int i, cplx<T> v) { R[i] = v; });
         } else {                // columns b0 .. b0 + RB - 1, each line down a column
-            for (int i = threadIdx.x; i < N * RB; i += BS) lds[(i / RB) * BP + i % RB] = A[(size_t)(i / RB) * N + b0 + i % RB];
+            cplx<T> *Cb = A + b0;
+            chunked_copy<T, N * RB, BS>(threadIdx.x, [&](int i) { return Cb[(size_t)(i / RB) * N + i % RB]; },
+                                        [&](int i, cplx<T> v) { lds[(i / RB) * BP + i % RB] = v; });
             __syncthreads();
-            line_op<T, N, -1, 1, BP>(lds, RB, d.twx, tab, threadIdx.x, BS);
-            for (int i = threadIdx.x; i < N * RB; i += BS) A[(size_t)(i / RB) * N + b0 + i % RB] = lds[(i / RB) * BP + i % RB];
+            band_line_op<T, N, -1, 1, BP>(lds, RB, d.twx, tab, threadIdx.x, BS);
+            chunked_copy<T, N * RB, BS>(threadIdx.x, [&](int i) { return lds[(i / RB) * BP + i % RB]; },
+                                        [&](int i, cplx<T> v) { Cb[(size_t)(i / RB) * N + i % RB] = v; });
         }
         __syncthreads();
     }
@@ -261,7 +267,7 @@ __global__ void __launch_bounds__(256) kb_power_final(const SolveDesc<T> *__rest
 bool batch_supported(int nx, int ny, int nt, bool fp64) {
     if (nx != ny || !tline_supported(nt, fp64) || nt < 16) return false;
     switch (nx) {
-        case 1: case 16: case 32: case 64: case 96: case 128: case 256: return true;
+        case 1: case 16: case 32: case 64: case 96: case 128: case 256: case 512: return true;
         default: return false;
     }
 }
@@ -359,6 +365,7 @@ hipError_t launch_batch_xy(hipStream_t st, int nx, int n_work, const SolveDesc<T
             if constexpr (sizeof(T) == 4) return bx_dispatch<T, 128>(st, n_work, sd, ts, status, work, free_space, variant);
             else return bx_band_variant<T, 128>(st, n_work, sd, ts, status, work, free_space, variant);
         case 256: return bx_band_variant<T, 256>(st, n_work, sd, ts, status, work, free_space, variant);
+        case 512: return bx_band_launch<T, 512, 16, 256>(st, n_work, sd, ts, status, work, free_space);
         default: return hipErrorInvalidValue;
     }
 }

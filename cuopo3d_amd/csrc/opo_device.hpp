@@ -569,6 +569,44 @@ __device__ __forceinline__ void line_op_any(const LineSet<T> &ls, int nlines, co
     else line_op_set<T, N, S, LS, ES, PAD, UNI>(ls, nlines, tw, tid, nthr);
 }
 
+// Copy of TOT elements by BS threads, element j of thread tid being
+// tid + j BS: ld(e) reads element e, st(e, v) writes it.  The reads of a
+// chunk of up to 32 VGPRs of elements are all issued before its first
+// write, so a thread waits on one memory latency per chunk rather than per
+// element (a plain strided loop waits after every load).
+template <typename T, int TOT, int BS, typename Ld, typename St>
+__device__ __forceinline__ void chunked_copy(int tid, Ld ld, St st) {
+    constexpr int IT = (TOT + BS - 1) / BS;
+    constexpr int CH = 32 / (int)(sizeof(cplx<T>) / 4);   // elements per chunk
+    constexpr bool EVEN = TOT % BS == 0;
+#pragma unroll
+    for (int c = 0; c < IT; c += CH) {
+        cplx<T> r[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int e = tid + (c + k) * BS;
+            if (c + k < IT && (EVEN || e < TOT)) r[k] = ld(e);
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int e = tid + (c + k) * BS;
+            if (c + k < IT && (EVEN || e < TOT)) st(e, r[k]);
+        }
+    }
+}
+
+// Lines of the row-band and column-band kernels (planes beyond the LDS):
+// 512-point lines take the three-level 8 x 8 x 8 operator in both
+// precisions (FP32 too: its two-level 32 x 16 split holds ~215 VGPRs).
+template <typename T, int N> constexpr bool band_three_level() { return three_level<T, N>() || N == 512; }
+template <typename T, int N, int S, int LS, int ES>
+__device__ __forceinline__ void band_line_op(cplx<T> *lds, int nlines, const cplx<T> *__restrict__ tw,
+                                             const cplx<T> *__restrict__ tab, int tid, int nthr) {
+    LineSet<T> ls{lds, 0, tab, 0, 1};
+    if constexpr (band_three_level<T, N>()) line_op_set3<T, N, S, LS, ES, 0, 0>(ls, nlines, tw, tid, nthr);
+    else line_op_set<T, N, S, LS, ES, 0, false>(ls, nlines, tw, tid, nthr);
+}
+
 // One field.
 template <typename T, int N, int S, int LS, int ES, bool UNI = false>
 __device__ __forceinline__ void line_op(cplx<T> *lds, int nlines, const cplx<T> *__restrict__ tw,

@@ -83,16 +83,12 @@ __global__ void __launch_bounds__(BS) k_tline(FieldOps<T> ops, int nx, int ny, c
     cplx<T> *A = ops.f[f];
     const size_t plane = (size_t)nx * ny;
     const size_t base = (size_t)y * nx + x0;
-    for (int i = tid; i < NT * TX; i += BS) {
-        const int t = i / TX, xl = i % TX;
-        lds[i] = A[(size_t)t * plane + base + xl];
-    }
+    chunked_copy<T, NT * TX, BS>(tid, [&](int i) { return A[(size_t)(i / TX) * plane + base + i % TX]; },
+                                 [&](int i, cplx<T> v) { lds[i] = v; });
     __syncthreads();
     line_op<T, NT, +1, 1, TX>(lds, TX, tw, ops.tab[f], tid, BS);
-    for (int i = tid; i < NT * TX; i += BS) {
-        const int t = i / TX, xl = i % TX;
-        A[(size_t)t * plane + base + xl] = lds[i];
-    }
+    chunked_copy<T, NT * TX, BS>(tid, [&](int i) { return lds[i]; },
+                                 [&](int i, cplx<T> v) { A[(size_t)(i / TX) * plane + base + i % TX] = v; });
 }
 
 template <typename T, int NT>
@@ -186,10 +182,10 @@ struct TuneTables {
     std::atomic<int> time[2][13];   // [fp64][log2 NT]
     std::atomic<int> xy[2][8];      // [fp64][NX index]
     TuneTables() {
-        const int nxs[7] = {1, 16, 32, 64, 96, 128, 256};
+        const int nxs[8] = {1, 16, 32, 64, 96, 128, 256, 512};
         for (int f = 0; f < 2; ++f) {
             for (int k = 0; k < 13; ++k) time[f][k].store(time_default_variant(f != 0, 1 << k));
-            for (int k = 0; k < 8; ++k) xy[f][k].store(k < 7 ? xy_default_variant(nxs[k]) : 0);
+            for (int k = 0; k < 8; ++k) xy[f][k].store(nxs[k] == 512 ? 0 : xy_default_variant(nxs[k]));
         }
     }
 };
@@ -206,7 +202,7 @@ static int log2i(int n) {
 static int nx_index(int nx) {
     switch (nx) {
         case 1: return 0; case 16: return 1; case 32: return 2; case 64: return 3;
-        case 96: return 4; case 128: return 5; case 256: return 6; default: return -1;
+        case 96: return 4; case 128: return 5; case 256: return 6; case 512: return 7; default: return -1;
     }
 }
 
@@ -257,7 +253,7 @@ bool tune_variant_valid(bool fp64, int axis, int n, int v) {
         return fp64 ? time_info<double>(n, v, tx, bs) : time_info<float>(n, v, tx, bs);
     }
     if (v < 0 || v >= kXyVariants || nx_index(n) < 0) return false;
-    if (n < 16) return v == 0;
+    if (n < 16 || n == 512) return v == 0;   // 512: the two-pass form, one band shape
     if (n == 256 || (fp64 && n == 128)) return v < kSplitVariants;   // the two-pass form
     if (n <= 32) return v < 6;
     if (xy_variant_persistent(n, v)) return !fp64 && n * n / xy_variant_bs(n, v) <= 16;
@@ -411,10 +407,12 @@ __global__ void __launch_bounds__(BS) k_xrows(FieldOps<T> ops, int ny, const cpl
     const int bands = ny / RB;
     const int t = blockIdx.x / bands, r0 = (blockIdx.x % bands) * RB, f = blockIdx.y;
     cplx<T> *A = ops.f[f] + ((size_t)t * ny + r0) * NX;
-    for (int i = threadIdx.x; i < RB * NX; i += BS) lds[(i / NX) * NXP + (i % NX)] = A[i];
+    chunked_copy<T, RB * NX, BS>(threadIdx.x, [&](int i) { return A[i]; },
+                                 [&](int i, cplx<T> v) { lds[(i / NX) * NXP + (i % NX)] = v; });
     __syncthreads();
-    line_op<T, NX, -1, NXP, 1>(lds, RB, tw, ops.tab[f], threadIdx.x, BS);
-    for (int i = threadIdx.x; i < RB * NX; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
+    band_line_op<T, NX, -1, NXP, 1>(lds, RB, tw, ops.tab[f], threadIdx.x, BS);
+    chunked_copy<T, RB * NX, BS>(threadIdx.x, [&](int i) { return lds[(i / NX) * NXP + (i % NX)]; },
+                                 [&](int i, cplx<T> v) { A[i] = v; });
 }
 
 template <typename T, int NY, int CB, int BS>
@@ -426,11 +424,13 @@ __global__ void __launch_bounds__(BS) k_ycols(FieldOps<T> ops, int nx, const cpl
     const int bands = nx / CB;
     const int t = blockIdx.x / bands, c0 = (blockIdx.x % bands) * CB, f = blockIdx.y;
     cplx<T> *A = ops.f[f] + (size_t)t * NY * nx + c0;
-    for (int i = threadIdx.x; i < NY * CB; i += BS) lds[(i / CB) * CBP + (i % CB)] = A[(size_t)(i / CB) * nx + (i % CB)];
+    chunked_copy<T, NY * CB, BS>(threadIdx.x, [&](int i) { return A[(size_t)(i / CB) * nx + (i % CB)]; },
+                                 [&](int i, cplx<T> v) { lds[(i / CB) * CBP + (i % CB)] = v; });
     __syncthreads();
     // lines are the CB columns (line c starts at c, its elements CBP apart)
-    line_op<T, NY, -1, 1, CBP>(lds, CB, tw, ops.taby[f], threadIdx.x, BS);
-    for (int i = threadIdx.x; i < NY * CB; i += BS) A[(size_t)(i / CB) * nx + (i % CB)] = lds[(i / CB) * CBP + (i % CB)];
+    band_line_op<T, NY, -1, 1, CBP>(lds, CB, tw, ops.taby[f], threadIdx.x, BS);
+    chunked_copy<T, NY * CB, BS>(threadIdx.x, [&](int i) { return lds[(i / CB) * CBP + (i % CB)]; },
+                                 [&](int i, cplx<T> v) { A[(size_t)(i / CB) * nx + (i % CB)] = v; });
 }
 
 // Bands of 32 rows or columns: 256 line tasks per phase of a 128-point
@@ -506,7 +506,7 @@ static hipError_t xy_dispatch(hipStream_t st, const Geometry &g, const FieldOps<
 
 static bool line_len_supported(int n) {
     switch (n) {
-        case 16: case 32: case 64: case 96: case 128: case 256: return true;
+        case 16: case 32: case 64: case 96: case 128: case 256: case 512: return true;
         default: return false;
     }
 }
@@ -521,32 +521,41 @@ bool xyplane_supported(int nx, int ny, bool fp64) {
 // dimension allows, else 16 (every supported length is a multiple of 16).
 // 256-point lines take the fastest square-plane band shapes: FP32 bands of
 // 32 on 512 threads (one 16-point DFT per thread and phase), FP64 bands of
-// 16 (64 KiB of LDS, two workgroups per CU).
-template <typename T, int N> constexpr int max_band() { return (sizeof(T) == 8 && N == 256) ? 16 : 32; }
-template <typename T, int N> constexpr int band32_threads() { return (sizeof(T) == 4 && N == 256) ? 512 : 256; }
+// 16 (64 KiB of LDS, two workgroups per CU).  512-point lines take bands
+// of 16 (FP32 66 KiB, FP64 131 KiB of LDS).
+template <typename T, int N> constexpr int max_band() {
+    return (sizeof(T) == 8 && N >= 256) ? 16 : 32;
+}
+template <typename T, int N> constexpr int band32_threads() { return (sizeof(T) == 4 && N >= 256) ? 512 : 256; }
 
 template <typename T, int NX>
 static hipError_t rows_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                               const cplx<T> *tw, const int *status) {
-    constexpr int B32 = band32_threads<T, NX>();
-    if (max_band<T, NX>() == 32 && g.ny % 32 == 0)
-        hipLaunchKernelGGL((k_xrows<T, NX, 32, B32>), dim3((unsigned)(g.nt * (g.ny / 32)), (unsigned)nf), dim3(B32),
-                           0, st, ops, g.ny, tw, status);
-    else
-        hipLaunchKernelGGL((k_xrows<T, NX, 16, 256>), dim3((unsigned)(g.nt * (g.ny / 16)), (unsigned)nf), dim3(256),
-                           0, st, ops, g.ny, tw, status);
+    if constexpr (max_band<T, NX>() == 32) {
+        constexpr int B32 = band32_threads<T, NX>();
+        if (g.ny % 32 == 0) {
+            hipLaunchKernelGGL((k_xrows<T, NX, 32, B32>), dim3((unsigned)(g.nt * (g.ny / 32)), (unsigned)nf),
+                               dim3(B32), 0, st, ops, g.ny, tw, status);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_xrows<T, NX, 16, 256>), dim3((unsigned)(g.nt * (g.ny / 16)), (unsigned)nf), dim3(256),
+                       0, st, ops, g.ny, tw, status);
     return hipGetLastError();
 }
 template <typename T, int NY>
 static hipError_t cols_launch(hipStream_t st, const Geometry &g, const FieldOps<T> &ops, int nf,
                               const cplx<T> *tw, const int *status) {
-    constexpr int B32 = band32_threads<T, NY>();
-    if (max_band<T, NY>() == 32 && g.nx % 32 == 0)
-        hipLaunchKernelGGL((k_ycols<T, NY, 32, B32>), dim3((unsigned)(g.nt * (g.nx / 32)), (unsigned)nf), dim3(B32),
-                           0, st, ops, g.nx, tw, status);
-    else
-        hipLaunchKernelGGL((k_ycols<T, NY, 16, 256>), dim3((unsigned)(g.nt * (g.nx / 16)), (unsigned)nf), dim3(256),
-                           0, st, ops, g.nx, tw, status);
+    if constexpr (max_band<T, NY>() == 32) {
+        constexpr int B32 = band32_threads<T, NY>();
+        if (g.nx % 32 == 0) {
+            hipLaunchKernelGGL((k_ycols<T, NY, 32, B32>), dim3((unsigned)(g.nt * (g.nx / 32)), (unsigned)nf),
+                               dim3(B32), 0, st, ops, g.nx, tw, status);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_ycols<T, NY, 16, 256>), dim3((unsigned)(g.nt * (g.nx / 16)), (unsigned)nf), dim3(256),
+                       0, st, ops, g.nx, tw, status);
     return hipGetLastError();
 }
 template <typename T>
@@ -562,6 +571,7 @@ static hipError_t xy_rect_launch(hipStream_t st, const Geometry &g, const FieldO
         case 96: e = rows_launch<T, 96>(st, g, ops, nf, tw_x, status); break;
         case 128: e = rows_launch<T, 128>(st, g, ops, nf, tw_x, status); break;
         case 256: e = rows_launch<T, 256>(st, g, ops, nf, tw_x, status); break;
+        case 512: e = rows_launch<T, 512>(st, g, ops, nf, tw_x, status); break;
     }
     if (e != hipSuccess) return e;
     switch (g.ny) {
@@ -571,6 +581,7 @@ static hipError_t xy_rect_launch(hipStream_t st, const Geometry &g, const FieldO
         case 96: return cols_launch<T, 96>(st, g, ops, nf, tw_y, status);
         case 128: return cols_launch<T, 128>(st, g, ops, nf, tw_y, status);
         case 256: return cols_launch<T, 256>(st, g, ops, nf, tw_y, status);
+        case 512: return cols_launch<T, 512>(st, g, ops, nf, tw_y, status);
         default: return hipErrorInvalidValue;
     }
 }
@@ -589,6 +600,7 @@ hipError_t launch_xyplane(hipStream_t st, const Geometry &g, const FieldOps<T> &
             if constexpr (sizeof(T) == 4) return xy_dispatch<T, 128>(st, g, ops, nf, tw, status);
             else return xy_split_launch<T, 128>(st, g, ops, nf, tw, status, tune_get(true, 1, 128));
         case 256: return xy_split_launch<T, 256>(st, g, ops, nf, tw, status, tune_get(sizeof(T) == 8, 1, 256));
+        case 512: return xy_rect_launch<T>(st, g, ops, nf, tw, tw_y, status);   // bands of 16, one shape
         default: return hipErrorInvalidValue;
     }
 }

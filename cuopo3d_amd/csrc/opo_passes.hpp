@@ -225,15 +225,32 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             lds[2 * FT + li + 1] = A1.i;
         }
     } else {
-        for (int i = tid; i < TILE; i += BS) {
-            const int t = i / TX, xl = i % TX;
-            const size_t gi = (size_t)t * plane + base + xl;
-            Tri<T> A = {a.psrc[gi], a.f[1][gi], a.f[2][gi]};
-            nl_point<T, PHASE>(rk, t, A);
-            const int li = L::at(i);
-            lds[li] = A.p;
-            lds[FT + li] = A.s;
-            lds[2 * FT + li] = A.i;
+        // one-pixel tiles: the loads of CH iterations are issued before the
+        // first of their pointwise steps, so a thread waits on one memory
+        // latency per CH iterations
+        constexpr int IT = (TILE + BS - 1) / BS, CH = sizeof(T) == 8 ? 4 : 8;
+#pragma unroll 1
+        for (int c = 0; c < IT; c += CH) {
+            Tri<T> A[CH];
+#pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                const int i = tid + (c + k) * BS;
+                if (c + k < IT && i < TILE) {
+                    const size_t gi = (size_t)(i / TX) * plane + base + i % TX;
+                    A[k] = {a.psrc[gi], a.f[1][gi], a.f[2][gi]};
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                const int i = tid + (c + k) * BS;
+                if (c + k < IT && i < TILE) {
+                    nl_point<T, PHASE>(rk, i / TX, A[k]);
+                    const int li = L::at(i);
+                    lds[li] = A[k].p;
+                    lds[FT + li] = A[k].s;
+                    lds[2 * FT + li] = A[k].i;
+                }
+            }
         }
     }
     if constexpr (STW) {
@@ -396,7 +413,8 @@ __device__ __forceinline__ void xy_plane(cplx<T> *A, const cplx<T> *__restrict__
                                          const cplx<T> *__restrict__ tw, cplx<T> *lds) {
     constexpr int NXP = NX + 1, NE = P * NX * NX;
     const int tid = threadIdx.x;
-    for (int i = tid; i < NE; i += BS) lds[(i / NX) * NXP + (i % NX)] = A[i];
+    chunked_copy<T, NE, BS>(tid, [&](int i) { return A[i]; },
+                            [&](int i, cplx<T> v) { lds[(i / NX) * NXP + (i % NX)] = v; });
     __syncthreads();
     constexpr bool UNI = xy_uniform<NX, BS>();
     // Q_x: lines are rows (pitch NXP), elements contiguous.
@@ -405,7 +423,8 @@ __device__ __forceinline__ void xy_plane(cplx<T> *A, const cplx<T> *__restrict__
     // Q_y: lines are columns, elements one row apart.
     const LineSet<T> ly{lds, NX * NXP, taby, 0, P};
     line_op_any<T, NX, -1, 1, NXP, 0, UNI>(ly, NX, tw, tid, BS);
-    for (int i = tid; i < NE; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
+    chunked_copy<T, NE, BS>(tid, [&](int i) { return lds[(i / NX) * NXP + (i % NX)]; },
+                            [&](int i, cplx<T> v) { A[i] = v; });
 }
 
 // Persistent form of the xy pass: the workgroup walks a run of planes and

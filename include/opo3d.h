@@ -47,9 +47,10 @@ extern "C" {
 #define OPO_FP64 1   /* all-FP64 device path                                 */
 
 /* Supported axis lengths of the device kernels.  nx, ny in {16, 32, 64,
- * 96, 128, 256}, independently (the reference declares NX and NY separately,
- * DtypesConsts.cuh:28-31; NX != NY planes and 256 x 256 planes take two
- * passes through HBM for the xy operator instead of one), or nx = ny = 1; nt in {1, 16, 32, 64,
+ * 96, 128, 256, 512}, independently (the reference declares NX and NY
+ * separately, DtypesConsts.cuh:28-31; NX != NY planes and planes of 256
+ * and 512 points take two passes through HBM for the xy operator instead
+ * of one), or nx = ny = 1; nt in {1, 16, 32, 64,
  * 128, 256, 512, 1024, 2048}, and 4096 on the FP32 path (three fields of
  * one pixel's 4096-sample line fill 102 KiB of LDS; FP64 would need twice
  * that).  A length of 1 turns the axis off (NX = NY =
@@ -276,7 +277,7 @@ void opo_batch_destroy(opo_batch *b);
 /* Queue a solve for n_trips round trips; fields are host buffers (copied).
  * Solves of one batch may differ in grid, slice count (crystal length, NZ)
  * and cavity parameters; they share the batch's precision.  Grids:
- * NX = NY in {16, 32, 64, 96, 128, 256}, or 1 x 1 with no_diffraction
+ * NX = NY in {16, 32, 64, 96, 128, 256, 512}, or 1 x 1 with no_diffraction
  * (the plane-wave model of Solver1D.cuh, modulator included); NT a power
  * of two from 16 to 2048 (4096 in FP32).  Solves with the dk phase term
  * (dk_phase_scale != 0) are refused: the reference multiplies it by 0

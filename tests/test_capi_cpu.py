@@ -99,6 +99,8 @@ def test_tune_set_rejects_missing_variants():
     assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 48, 0) != capi.OPO_OK    # no such NX
     assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 256, 3) != capi.OPO_OK   # 256: three band shapes
     assert lib.opo_tune_get(OPO_FP32, OPO_TUNE_XY_PASS, 256) in (0, 1, 2)
+    assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_XY_PASS, 512, 1) != capi.OPO_OK   # 512: one band shape
+    assert lib.opo_tune_get(OPO_FP32, OPO_TUNE_XY_PASS, 512) == 0
     assert lib.opo_tune_set(OPO_FP32, OPO_TUNE_TIME_PASS, 256, keep) == capi.OPO_OK
     assert lib.opo_tune_get(OPO_FP32, OPO_TUNE_TIME_PASS, 256) == keep
 

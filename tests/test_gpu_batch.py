@@ -275,14 +275,14 @@ def test_fp64_batch_equals_single_solves(require_gpu):
         assert max(errs) < 1e-11
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp64"])
-def test_batch_256_planes_equal_single_solves(prec, require_gpu):
-    """A 256 x 256 solve (row-band then column-band xy passes) batched with
-    a 32 x 32 one gives the single-solve fields and per-trip power bitwise,
-    in either precision."""
+@pytest.mark.parametrize("prec,nxy", [("fp32", 256), ("fp64", 256), ("fp32", 512), ("fp64", 512)])
+def test_batch_large_planes_equal_single_solves(prec, nxy, require_gpu):
+    """A 256 x 256 or 512 x 512 solve (row-band then column-band xy
+    passes) batched with a 32 x 32 one gives the single-solve fields and
+    per-trip power bitwise, in either precision."""
     from cuopo3d_amd.capi import OPO_FP64
     pr = OPO_FP64 if prec == "fp64" else OPO_FP32
-    solves = [configs.cavity_64(n=n, n_round_trips=2, precision=pr) for n in ((256, 256, 16), (32, 32, 64))]
+    solves = [configs.cavity_64(n=n, n_round_trips=2, precision=pr) for n in ((nxy, nxy, 16), (32, 32, 64))]
     b = co.Batch(4 * GB, precision=pr)
     tickets = [b.submit(g, p, *f, 2) for (g, p, ph, f) in solves]
     res = {}
@@ -306,7 +306,7 @@ def test_batch_256_planes_equal_single_solves(prec, require_gpu):
             assert np.array_equal(a, c), (prec, g.nx)
         assert np.array_equal(ps, pb), (prec, g.nx)
         s.close()
-    print(prec, "256 x 256 batch == single solve", flush=True)
+    print(prec, nxy, "batch == single solve", flush=True)
 
 
 def test_batch_concurrent_classes_bitwise(require_gpu):

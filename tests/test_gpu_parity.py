@@ -17,7 +17,7 @@ from helpers import rel_l2, random_fields, cavity_params
 
 pytestmark = pytest.mark.gpu
 
-OP_GRIDS = [(16, 16, 16), (32, 32, 64), (64, 64, 32), (96, 96, 16), (128, 128, 16), (256, 256, 16),
+OP_GRIDS = [(16, 16, 16), (32, 32, 64), (64, 64, 32), (96, 96, 16), (128, 128, 16), (256, 256, 16), (512, 512, 16),
             (16, 16, 128), (16, 16, 256), (16, 16, 512), (16, 16, 1024), (16, 16, 2048), (16, 16, 4096),
             (1, 1, 64)]
 
@@ -81,7 +81,7 @@ def test_4096_sample_lines_against_oracle(n, require_gpu):
 
 
 # NX != NY: Q_x on bands of rows and Q_y on bands of columns (two passes)
-RECT_GRIDS = [(64, 32, 16), (32, 96, 16), (128, 16, 16), (16, 128, 32), (256, 32, 16), (64, 256, 16)]
+RECT_GRIDS = [(64, 32, 16), (32, 96, 16), (128, 16, 16), (16, 128, 32), (256, 32, 16), (64, 256, 16), (512, 32, 16)]
 
 
 @pytest.mark.parametrize("n", RECT_GRIDS, ids=lambda n: "x".join(map(str, n)))
@@ -93,7 +93,7 @@ def test_diffraction_operator_rectangular(n, require_gpu):
     _check(s, o, 1e-6, f"Q {n}")
 
 
-@pytest.mark.parametrize("n", [(128, 64, 16), (32, 128, 16), (256, 256, 16), (256, 64, 16)],
+@pytest.mark.parametrize("n", [(128, 64, 16), (32, 128, 16), (256, 256, 16), (256, 64, 16), (512, 512, 16)],
                          ids=lambda n: "x".join(map(str, n)))
 def test_diffraction_operator_rectangular_fp64(n, require_gpu):
     grid, p, _ = cavity_params(n, precision=OPO_FP64)
@@ -103,8 +103,8 @@ def test_diffraction_operator_rectangular_fp64(n, require_gpu):
     _check(s, o, 1e-12, f"Q fp64 {n}")
 
 
-@pytest.mark.parametrize("n,prec", [((64, 32, 128), OPO_FP32), ((32, 96, 64), OPO_FP64)],
-                         ids=["64x32x128", "32x96x64-fp64"])
+@pytest.mark.parametrize("n,prec", [((64, 32, 128), OPO_FP32), ((32, 96, 64), OPO_FP64), ((512, 64, 16), OPO_FP32)],
+                         ids=["64x32x128", "32x96x64-fp64", "512x64x16"])
 def test_rectangular_cavity_against_oracle(n, prec, require_gpu):
     """A cavity on an NX != NY grid (the reference declares NX and NY
     separately, DtypesConsts.cuh:28-31): 2 trips against the oracle, fields
@@ -120,7 +120,8 @@ def test_rectangular_cavity_against_oracle(n, prec, require_gpu):
     assert rel[:, 1].max() < (1e-6 if prec == OPO_FP32 else 1e-11), rel
 
 
-@pytest.mark.parametrize("n", [(32, 32, 64), (128, 128, 16), (64, 32, 64)], ids=lambda n: "x".join(map(str, n)))
+@pytest.mark.parametrize("n", [(32, 32, 64), (128, 128, 16), (64, 32, 64), (512, 512, 16)],
+                         ids=lambda n: "x".join(map(str, n)))
 def test_free_space_and_gdd_and_mirror(n, require_gpu):
     grid, p, _ = cavity_params(n, rs=0.9, phase_s=0.7)
     s, o = _pair(grid, p, random_fields(grid, 13))
