@@ -573,14 +573,20 @@ __device__ __forceinline__ void line_op_any(const LineSet<T> &ls, int nlines, co
 // tid + j BS: ld(e) reads element e, st(e, v) writes it.  The reads of a
 // chunk of up to 32 VGPRs of elements are all issued before its first
 // write, so a thread waits on one memory latency per chunk rather than per
-// element (a plain strided loop waits after every load).
+// element (a plain strided loop waits after every load).  Threads with
+// fewer than 8 elements keep the plain loop: their few waits cost less than
+// the registers (FP64 32 x 32 planes measured 20 % slower chunked).
 template <typename T, int TOT, int BS, typename Ld, typename St>
 __device__ __forceinline__ void chunked_copy(int tid, Ld ld, St st) {
     constexpr int IT = (TOT + BS - 1) / BS;
     constexpr int CH = 32 / (int)(sizeof(cplx<T>) / 4);   // elements per chunk
     constexpr bool EVEN = TOT % BS == 0;
-#pragma unroll
-    for (int c = 0; c < IT; c += CH) {
+    if constexpr (IT < 8) {
+        for (int e = tid; e < TOT; e += BS) st(e, ld(e));
+        return;
+    }
+#pragma unroll 1
+    for (int c = 0; c < IT; c += CH) {   // kept rolled: one chunk of registers live at a time
         cplx<T> r[CH];
 #pragma unroll
         for (int k = 0; k < CH; ++k) {

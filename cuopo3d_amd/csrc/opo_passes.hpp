@@ -225,32 +225,15 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             lds[2 * FT + li + 1] = A1.i;
         }
     } else {
-        // one-pixel tiles: the loads of CH iterations are issued before the
-        // first of their pointwise steps, so a thread waits on one memory
-        // latency per CH iterations
-        constexpr int IT = (TILE + BS - 1) / BS, CH = sizeof(T) == 8 ? 4 : 8;
-#pragma unroll 1
-        for (int c = 0; c < IT; c += CH) {
-            Tri<T> A[CH];
-#pragma unroll
-            for (int k = 0; k < CH; ++k) {
-                const int i = tid + (c + k) * BS;
-                if (c + k < IT && i < TILE) {
-                    const size_t gi = (size_t)(i / TX) * plane + base + i % TX;
-                    A[k] = {a.psrc[gi], a.f[1][gi], a.f[2][gi]};
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < CH; ++k) {
-                const int i = tid + (c + k) * BS;
-                if (c + k < IT && i < TILE) {
-                    nl_point<T, PHASE>(rk, i / TX, A[k]);
-                    const int li = L::at(i);
-                    lds[li] = A[k].p;
-                    lds[FT + li] = A[k].s;
-                    lds[2 * FT + li] = A[k].i;
-                }
-            }
+        for (int i = tid; i < TILE; i += BS) {
+            const int t = i / TX, xl = i % TX;
+            const size_t gi = (size_t)t * plane + base + xl;
+            Tri<T> A = {a.psrc[gi], a.f[1][gi], a.f[2][gi]};
+            nl_point<T, PHASE>(rk, t, A);
+            const int li = L::at(i);
+            lds[li] = A.p;
+            lds[FT + li] = A.s;
+            lds[2 * FT + li] = A.i;
         }
     }
     if constexpr (STW) {
