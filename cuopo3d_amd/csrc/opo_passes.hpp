@@ -119,6 +119,28 @@ template <typename T> struct alignas(4 * sizeof(T)) cpair {
     cplx<T> a, b;
 };
 
+// Two consecutive cells to and from memory.  FP32: one 16-byte access.
+// FP64: element by element; a cpair<double> or cplx<double> copied as a
+// struct between memory spaces goes through a private-memory temporary
+// (scratch) in the fused-pass kernels, so FP64 values move component-wise.
+template <typename T> __device__ __forceinline__ cplx<T> ld_c(const cplx<T> *p) { return cplx<T>{p->x, p->y}; }
+template <typename T> __device__ __forceinline__ void st_c(cplx<T> *p, cplx<T> v) {
+    p->x = v.x;
+    p->y = v.y;
+}
+template <typename T> __device__ __forceinline__ cpair<T> load_pair(const cplx<T> *p) {
+    if constexpr (sizeof(T) == 8) return cpair<T>{ld_c(p), ld_c(p + 1)};
+    else return *reinterpret_cast<const cpair<T> *>(p);
+}
+template <typename T> __device__ __forceinline__ void store_pair(cplx<T> *p, const cplx<T> *q) {
+    if constexpr (sizeof(T) == 8) {
+        st_c(p, ld_c(q));
+        st_c(p + 1, ld_c(q + 1));
+    } else {
+        *reinterpret_cast<cpair<T> *>(p) = cpair<T>{q[0], q[1]};
+    }
+}
+
 // Pointwise part of the fused pass for two cells: RK4 half step(s), then
 // the mirror factors on s and i when the pass closes a round trip.
 template <typename T, bool PHASE>
@@ -196,7 +218,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     if constexpr (STW) {
 #pragma unroll
         for (int j = 0; j < TWP; ++j)
-            if (tid + j * BS < 2 * NT) twr[j] = tw[tid + j * BS];
+            if (tid + j * BS < 2 * NT) twr[j] = ld_c(tw + tid + j * BS);
     }
     const int tiles_x = nx / TX;
     const int y = tile / tiles_x, x0 = (tile % tiles_x) * TX;
@@ -211,9 +233,9 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
-            cpair<T> p = *reinterpret_cast<const cpair<T> *>(a.psrc + gi);
-            cpair<T> s = *reinterpret_cast<const cpair<T> *>(a.f[1] + gi);
-            cpair<T> q = *reinterpret_cast<const cpair<T> *>(a.f[2] + gi);
+            cpair<T> p = load_pair(a.psrc + gi);
+            cpair<T> s = load_pair(a.f[1] + gi);
+            cpair<T> q = load_pair(a.f[2] + gi);
             Tri<T> A0 = {p.a, s.a, q.a}, A1 = {p.b, s.b, q.b};
             nl_point_pair<T, PHASE>(rk, t, A0, A1);
             const int li = L::at(i);
@@ -228,7 +250,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
         for (int i = tid; i < TILE; i += BS) {
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
-            Tri<T> A = {a.psrc[gi], a.f[1][gi], a.f[2][gi]};
+            Tri<T> A = {ld_c(a.psrc + gi), ld_c(a.f[1] + gi), ld_c(a.f[2] + gi)};
             nl_point<T, PHASE>(rk, t, A);
             const int li = L::at(i);
             lds[li] = A.p;
@@ -239,7 +261,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     if constexpr (STW) {
 #pragma unroll
         for (int j = 0; j < TWP; ++j)
-            if (tid + j * BS < 2 * NT) twl[tid + j * BS] = twr[j];
+            if (tid + j * BS < 2 * NT) st_c(twl + tid + j * BS, twr[j]);
     }
     __syncthreads();
     const LineSet<T> ls{lds + a.first_field * FT, FT, a.tab0, a.tab_stride, 3 - a.first_field};
@@ -249,15 +271,15 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
             const int li = L::at(i);
-            *reinterpret_cast<cpair<T> *>(a.f[0] + gi) = cpair<T>{lds[li], lds[li + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[1] + gi) = cpair<T>{lds[FT + li], lds[FT + li + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[2] + gi) = cpair<T>{lds[2 * FT + li], lds[2 * FT + li + 1]};
+            store_pair(a.f[0] + gi, lds + li);
+            store_pair(a.f[1] + gi, lds + FT + li);
+            store_pair(a.f[2] + gi, lds + 2 * FT + li);
         }
     } else {
         for (int i = tid; i < TILE; i += BS) {
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
-            for (int f = 0; f < 3; ++f) a.f[f][gi] = lds[f * FT + L::at(i)];
+            for (int f = 0; f < 3; ++f) st_c(a.f[f] + gi, ld_c(lds + f * FT + L::at(i)));
         }
     }
 }
