@@ -123,6 +123,51 @@ template <int NX, bool ROWS, bool UNI> void xy_plane(const char *label) {
     std::printf("%s %.6e\n", label, rel(out, c.ref));
 }
 
+// Band layout of the row-band and column-band kernels: RB lines of N
+// points, rows of pitch N + 1 (ROWS) or columns of a band of RB + 1 pitch.
+template <int N, int RB, bool ROWS> void band(const char *label) {
+    constexpr int NP = N + 1, BP = RB + 1;
+    Case c = make<N, -1>(RB, 1, 11 * N + RB + ROWS);
+    std::vector<cplx<float>> lds((size_t)(ROWS ? RB * NP : N * BP));
+    auto at = [](int l, int n) { return ROWS ? l * NP + n : n * BP + l; };
+    for (int l = 0; l < RB; ++l)
+        for (int n = 0; n < N; ++n) {
+            const cd v = c.x[(size_t)l * N + n];
+            lds[at(l, n)] = {(float)v.real(), (float)v.imag()};
+        }
+    if constexpr (ROWS) band_line_op<float, N, -1, NP, 1>(lds.data(), RB, c.tw.data(), c.tabf.data(), 0, 1);
+    else band_line_op<float, N, -1, 1, BP>(lds.data(), RB, c.tw.data(), c.tabf.data(), 0, 1);
+    std::vector<cd> out(c.x.size());
+    for (int l = 0; l < RB; ++l)
+        for (int n = 0; n < N; ++n) {
+            const cplx<float> v = lds[at(l, n)];
+            out[(size_t)l * N + n] = cd(v.x, v.y);
+        }
+    std::printf("%s %.6e\n", label, rel(out, c.ref));
+}
+
+// chunked_copy run for every thread of a BS-thread workgroup in turn:
+// prints the number of elements of [0, TOT) not copied exactly once plus
+// the number of accesses outside it (0 when the index map is a bijection).
+template <typename T, int TOT, int BS> void copy_cover(const char *label) {
+    std::vector<int> rd(TOT, 0), wr(TOT, 0);
+    long bad = 0;
+    for (int tid = 0; tid < BS; ++tid)
+        chunked_copy<T, TOT, BS>(
+            tid,
+            [&](int e) {
+                if (e < 0 || e >= TOT) { ++bad; return cplx<T>{}; }
+                ++rd[e];
+                return cplx<T>{(T)e, (T)-e};
+            },
+            [&](int e, cplx<T> v) {
+                if (e < 0 || e >= TOT || v.x != (T)e || v.y != (T)-e) { ++bad; return; }
+                ++wr[e];
+            });
+    for (int e = 0; e < TOT; ++e) bad += (rd[e] != 1) + (wr[e] != 1);
+    std::printf("%s %ld\n", label, bad);
+}
+
 int main() {
     time_tile<256, 4, +1, 2>("t256_two_level");
     time_tile<512, 4, +1, 2>("t512_two_level");
@@ -139,5 +184,15 @@ int main() {
     xy_plane<64, false, true>("x64_columns_uniform");
     xy_plane<96, false, false>("x96_columns");
     xy_plane<128, true, true>("x128_rows_uniform");
+    band<256, 32, true>("b256_rows");
+    band<256, 16, false>("b256_columns");
+    band<512, 16, true>("b512_rows_three_level");
+    band<512, 16, false>("b512_columns_three_level");
+    copy_cover<float, 512 * 32, 512>("copy_f32_16384_512");    // 32 per thread: two chunks
+    copy_cover<double, 512 * 16, 256>("copy_f64_8192_256");    // 32 per thread: four chunks
+    copy_cover<float, 96 * 96, 256>("copy_f32_9216_256");      // 36 per thread: last chunk partial
+    copy_cover<double, 96 * 96, 1024>("copy_f64_9216_1024");   // 9 per thread
+    copy_cover<float, 3 * 1000, 256>("copy_f32_3000_256");     // ragged: 12 per thread, tail masked
+    copy_cover<double, 32 * 32, 256>("copy_f64_1024_256");     // 4 per thread: the plain loop
     return 0;
 }
