@@ -187,20 +187,20 @@ __global__ void __launch_bounds__(BS) kb_xyband(const SolveDesc<T> *__restrict__
     for (int b0 = 0; b0 < N; b0 += RB) {
         if constexpr (ROWS) {   // rows b0 .. b0 + RB - 1, pitch N + 1
             cplx<T> *R = A + (size_t)b0 * N;
-            chunked_copy<T, RB * N, BS>(threadIdx.x, [&](int i) { return R[i]; },
+            chunked_copy<T, RB * N, BS>(threadIdx.x, [&](int i) { return gld_c(R + i); },
                                         [&](int i, cplx<T> v) { lds[(i / N) * (N + 1) + i % N] = v; });
             __syncthreads();
             band_line_op<T, N, -1, N + 1, 1>(lds, RB, d.twx, tab, threadIdx.x, BS);
             chunked_copy<T, RB * N, BS>(threadIdx.x, [&](int i) { return lds[(i / N) * (N + 1) + i % N]; },
-                                        [&](int i, cplx<T> v) { R[i] = v; });
+                                        [&](int i, cplx<T> v) { gst_c(R + i, v); });
         } else {                // columns b0 .. b0 + RB - 1, each line down a column
             cplx<T> *Cb = A + b0;
-            chunked_copy<T, N * RB, BS>(threadIdx.x, [&](int i) { return Cb[(size_t)(i / RB) * N + i % RB]; },
+            chunked_copy<T, N * RB, BS>(threadIdx.x, [&](int i) { return gld_c(Cb + (size_t)(i / RB) * N + i % RB); },
                                         [&](int i, cplx<T> v) { lds[(i / RB) * BP + i % RB] = v; });
             __syncthreads();
             band_line_op<T, N, -1, 1, BP>(lds, RB, d.twx, tab, threadIdx.x, BS);
             chunked_copy<T, N * RB, BS>(threadIdx.x, [&](int i) { return lds[(i / RB) * BP + i % RB]; },
-                                        [&](int i, cplx<T> v) { Cb[(size_t)(i / RB) * N + i % RB] = v; });
+                                        [&](int i, cplx<T> v) { gst_c(Cb + (size_t)(i / RB) * N + i % RB, v); });
         }
         __syncthreads();
     }

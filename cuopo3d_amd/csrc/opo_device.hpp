@@ -569,6 +569,23 @@ __device__ __forceinline__ void line_op_any(const LineSet<T> &ls, int nlines, co
     else line_op_set<T, N, S, LS, ES, PAD, UNI>(ls, nlines, tw, tid, nthr);
 }
 
+// Global-memory accesses through pointers whose address space the compiler
+// cannot see (field and table pointers read from a solve descriptor in
+// memory).  Left generic, they compile to flat instructions, which count
+// against the LDS wait counter too: every LDS wait of a line operator would
+// then also wait for the prefetch of the next tile or plane.  The pointer
+// must address global memory; values move component-wise (see ld_c).
+#define OPO_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ cplx<T> gld_c(const cplx<T> *p) {
+    const OPO_GLOBAL cplx<T> *q = (const OPO_GLOBAL cplx<T> *)p;
+    return cplx<T>{q->x, q->y};
+}
+template <typename T> __device__ __forceinline__ void gst_c(cplx<T> *p, cplx<T> v) {
+    OPO_GLOBAL cplx<T> *q = (OPO_GLOBAL cplx<T> *)p;
+    q->x = v.x;
+    q->y = v.y;
+}
+
 // Copy of TOT elements by BS threads, element j of thread tid being
 // tid + j BS: ld(e) reads element e, st(e, v) writes it.  The reads of a
 // chunk of up to 32 VGPRs of elements are all issued before its first

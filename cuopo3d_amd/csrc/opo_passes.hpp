@@ -119,26 +119,27 @@ template <typename T> struct alignas(4 * sizeof(T)) cpair {
     cplx<T> a, b;
 };
 
-// Two consecutive cells to and from memory.  FP32: one 16-byte access.
-// FP64: element by element; a cpair<double> or cplx<double> copied as a
+// Values move component-wise: a cpair<double> or cplx<double> copied as a
 // struct between memory spaces goes through a private-memory temporary
-// (scratch) in the fused-pass kernels, so FP64 values move component-wise.
+// (scratch) in the fused-pass kernels.
 template <typename T> __device__ __forceinline__ cplx<T> ld_c(const cplx<T> *p) { return cplx<T>{p->x, p->y}; }
 template <typename T> __device__ __forceinline__ void st_c(cplx<T> *p, cplx<T> v) {
     p->x = v.x;
     p->y = v.y;
 }
+// Two consecutive cells of a field in global memory (p 4 sizeof(T)-byte
+// aligned: an even cell index), to registers and back from LDS (q).
 template <typename T> __device__ __forceinline__ cpair<T> load_pair(const cplx<T> *p) {
-    if constexpr (sizeof(T) == 8) return cpair<T>{ld_c(p), ld_c(p + 1)};
-    else return *reinterpret_cast<const cpair<T> *>(p);
+    const OPO_GLOBAL cpair<T> *g = (const OPO_GLOBAL cpair<T> *)p;
+    return cpair<T>{cplx<T>{g->a.x, g->a.y}, cplx<T>{g->b.x, g->b.y}};
 }
 template <typename T> __device__ __forceinline__ void store_pair(cplx<T> *p, const cplx<T> *q) {
-    if constexpr (sizeof(T) == 8) {
-        st_c(p, ld_c(q));
-        st_c(p + 1, ld_c(q + 1));
-    } else {
-        *reinterpret_cast<cpair<T> *>(p) = cpair<T>{q[0], q[1]};
-    }
+    OPO_GLOBAL cpair<T> *g = (OPO_GLOBAL cpair<T> *)p;
+    const cplx<T> a = ld_c(q), b = ld_c(q + 1);
+    g->a.x = a.x;
+    g->a.y = a.y;
+    g->b.x = b.x;
+    g->b.y = b.y;
 }
 
 // Pointwise part of the fused pass for two cells: RK4 half step(s), then
@@ -218,7 +219,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     if constexpr (STW) {
 #pragma unroll
         for (int j = 0; j < TWP; ++j)
-            if (tid + j * BS < 2 * NT) twr[j] = ld_c(tw + tid + j * BS);
+            if (tid + j * BS < 2 * NT) twr[j] = gld_c(tw + tid + j * BS);
     }
     const int tiles_x = nx / TX;
     const int y = tile / tiles_x, x0 = (tile % tiles_x) * TX;
@@ -250,7 +251,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
         for (int i = tid; i < TILE; i += BS) {
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
-            Tri<T> A = {ld_c(a.psrc + gi), ld_c(a.f[1] + gi), ld_c(a.f[2] + gi)};
+            Tri<T> A = {gld_c(a.psrc + gi), gld_c(a.f[1] + gi), gld_c(a.f[2] + gi)};
             nl_point<T, PHASE>(rk, t, A);
             const int li = L::at(i);
             lds[li] = A.p;
@@ -279,7 +280,7 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
         for (int i = tid; i < TILE; i += BS) {
             const int t = i / TX, xl = i % TX;
             const size_t gi = (size_t)t * plane + base + xl;
-            for (int f = 0; f < 3; ++f) st_c(a.f[f] + gi, ld_c(lds + f * FT + L::at(i)));
+            for (int f = 0; f < 3; ++f) gst_c(a.f[f] + gi, ld_c(lds + f * FT + L::at(i)));
         }
     }
 }
@@ -328,9 +329,9 @@ __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T
             const int i = 2 * tid + j * 2 * BS;
             if (EVEN || i < TILE) {
                 const size_t gi = offset(u, i);
-                rp[j] = *reinterpret_cast<const cpair<T> *>(a.psrc + gi);
-                rs[j] = *reinterpret_cast<const cpair<T> *>(a.f[1] + gi);
-                ri[j] = *reinterpret_cast<const cpair<T> *>(a.f[2] + gi);
+                rp[j] = load_pair(a.psrc + gi);
+                rs[j] = load_pair(a.f[1] + gi);
+                ri[j] = load_pair(a.f[2] + gi);
             }
         }
     };
@@ -368,9 +369,9 @@ __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const size_t gi = offset(u, i);
             const int li = L::at(i);
-            *reinterpret_cast<cpair<T> *>(a.f[0] + gi) = cpair<T>{lds[li], lds[li + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[1] + gi) = cpair<T>{lds[FT + li], lds[FT + li + 1]};
-            *reinterpret_cast<cpair<T> *>(a.f[2] + gi) = cpair<T>{lds[2 * FT + li], lds[2 * FT + li + 1]};
+            store_pair(a.f[0] + gi, lds + li);
+            store_pair(a.f[1] + gi, lds + FT + li);
+            store_pair(a.f[2] + gi, lds + 2 * FT + li);
         }
         __syncthreads();
     }
@@ -381,9 +382,9 @@ __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T
 template <typename T, int NT>
 __device__ __forceinline__ void stage_time_tables(const cplx<T> *__restrict__ tw, const TLine3Args<T> &a,
                                                   cplx<T> *stab, int tid, int nthr) {
-    for (int i = tid; i < 2 * NT; i += nthr) stab[i] = tw[i];
+    for (int i = tid; i < 2 * NT; i += nthr) stab[i] = gld_c(tw + i);
     for (int f = 0; f < 3 - a.first_field; ++f)
-        for (int i = tid; i < 2 * NT; i += nthr) stab[2 * NT * (1 + f) + i] = a.tab0[(size_t)f * a.tab_stride + i];
+        for (int i = tid; i < 2 * NT; i += nthr) stab[2 * NT * (1 + f) + i] = gld_c(a.tab0 + (size_t)f * a.tab_stride + i);
 }
 
 // Work units of the fused pass of one solve: its tiles.
@@ -418,7 +419,7 @@ __device__ __forceinline__ void xy_plane(cplx<T> *A, const cplx<T> *__restrict__
                                          const cplx<T> *__restrict__ tw, cplx<T> *lds) {
     constexpr int NXP = NX + 1, NE = P * NX * NX;
     const int tid = threadIdx.x;
-    chunked_copy<T, NE, BS>(tid, [&](int i) { return A[i]; },
+    chunked_copy<T, NE, BS>(tid, [&](int i) { return gld_c(A + i); },
                             [&](int i, cplx<T> v) { lds[(i / NX) * NXP + (i % NX)] = v; });
     __syncthreads();
     constexpr bool UNI = xy_uniform<NX, BS>();
@@ -429,7 +430,7 @@ __device__ __forceinline__ void xy_plane(cplx<T> *A, const cplx<T> *__restrict__
     const LineSet<T> ly{lds, NX * NXP, taby, 0, P};
     line_op_any<T, NX, -1, 1, NXP, 0, UNI>(ly, NX, tw, tid, BS);
     chunked_copy<T, NE, BS>(tid, [&](int i) { return lds[(i / NX) * NXP + (i % NX)]; },
-                            [&](int i, cplx<T> v) { A[i] = v; });
+                            [&](int i, cplx<T> v) { gst_c(A + i, v); });
 }
 
 // Persistent form of the xy pass: the workgroup walks a run of planes and
@@ -456,7 +457,7 @@ __device__ __forceinline__ void xy_run(const Src &src, cplx<T> *lds, cplx<T> *st
 #pragma unroll
         for (int j = 0; j < PT; ++j) {
             const int i = tid + j * BS;
-            if (EVEN || i < NN) r[j] = A[i];
+            if (EVEN || i < NN) r[j] = gld_c(A + i);
         }
     };
     int staged = -1;
@@ -488,7 +489,7 @@ __device__ __forceinline__ void xy_run(const Src &src, cplx<T> *lds, cplx<T> *st
             line_op<T, NX, -1, 1, NXP>(lds, NX, stab, stab + 2 * NX * (4 + k), tid, BS);
         }
         cplx<T> *A = src.plane(u);
-        for (int i = tid; i < NN; i += BS) A[i] = lds[(i / NX) * NXP + (i % NX)];
+        for (int i = tid; i < NN; i += BS) gst_c(A + i, lds[(i / NX) * NXP + (i % NX)]);
         __syncthreads();
     }
 }
@@ -498,11 +499,11 @@ template <typename T, int NX>
 __device__ __forceinline__ void stage_xy_tables(const cplx<T> *__restrict__ tw, const cplx<T> *const *tx,
                                                 const cplx<T> *const *ty, int nf, cplx<T> *stab, int tid,
                                                 int nthr) {
-    for (int i = tid; i < 2 * NX; i += nthr) stab[i] = tw[i];
+    for (int i = tid; i < 2 * NX; i += nthr) stab[i] = gld_c(tw + i);
     for (int f = 0; f < nf; ++f)
         for (int i = tid; i < 2 * NX; i += nthr) {
-            stab[2 * NX * (1 + f) + i] = tx[f][i];
-            stab[2 * NX * (4 + f) + i] = ty[f][i];
+            stab[2 * NX * (1 + f) + i] = gld_c(tx[f] + i);
+            stab[2 * NX * (4 + f) + i] = gld_c(ty[f] + i);
         }
 }
 
