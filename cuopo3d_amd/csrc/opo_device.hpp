@@ -26,6 +26,24 @@ template <typename T> struct alignas(2 * sizeof(T)) cplx {
     T x, y;
 };
 
+// Global-memory accesses through pointers whose address space the compiler
+// cannot see (field and table pointers read from a solve descriptor in
+// memory).  Left generic, they compile to flat instructions, which count
+// against the LDS wait counter too: every LDS wait of a line operator would
+// then also wait for the prefetch of the next tile or plane.  The pointer
+// must address global memory; values move component-wise (see ld_c).
+#define OPO_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ cplx<T> gld_c(const cplx<T> *p) {
+    const OPO_GLOBAL cplx<T> *q = (const OPO_GLOBAL cplx<T> *)p;
+    return cplx<T>{q->x, q->y};
+}
+template <typename T> __device__ __forceinline__ void gst_c(cplx<T> *p, cplx<T> v) {
+    OPO_GLOBAL cplx<T> *q = (OPO_GLOBAL cplx<T> *)p;
+    q->x = v.x;
+    q->y = v.y;
+}
+
+
 // Every multiply-add is written as an explicit fused operation and the
 // library is compiled with floating-point contraction off, so each result
 // is one fixed sequence of IEEE operations: tile shapes, block sizes and
@@ -299,11 +317,14 @@ template <typename T> using const_space_ptr = const __attribute__((address_space
 // constant address space: the kernel never writes it, and a wave-uniform
 // index then becomes a scalar load even where the kernel stores to global
 // memory between two reads (which otherwise forces vector loads).
-template <typename T, bool UNI>
+// GT: the table is in global memory (see gld_c).
+template <typename T, bool UNI, bool GT = false>
 __device__ __forceinline__ cplx<T> table_at(const cplx<T> *__restrict__ p, int i) {
     if constexpr (UNI) {
         const const_space_ptr<T> q = (const_space_ptr<T>)p;
         return cplx<T>{q[2 * i], q[2 * i + 1]};
+    } else if constexpr (GT) {
+        return gld_c(p + i);
     } else {
         return p[i];
     }
@@ -323,7 +344,8 @@ __device__ __forceinline__ void phase1_regs(cplx<T> *v, int n2, const cplx<T> *_
     }
 }
 
-template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false>
+// GT: the line tables are in global memory (twiddles may be staged in LDS).
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false, bool GT = false>
 __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
                                             const cplx<T> *__restrict__ tw, int tid, int nthr) {
     constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
@@ -340,7 +362,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
             for (int n = 0; n < N; ++n) v[n] = base[n * ES];
             dft<N, S>(v);
 #pragma unroll
-            for (int k = 0; k < N; ++k) v[k] = cmul_hl(v[k], table_at<T, UNI>(tab, 2 * k), table_at<T, UNI>(tab, 2 * k + 1));
+            for (int k = 0; k < N; ++k) v[k] = cmul_hl(v[k], table_at<T, UNI, GT>(tab, 2 * k), table_at<T, UNI, GT>(tab, 2 * k + 1));
             dft<N, -S>(v);
 #pragma unroll
             for (int n = 0; n < N; ++n) base[n * ES] = v[n];
@@ -373,7 +395,7 @@ __device__ __forceinline__ void line_op_set(const LineSet<T> &ls, int nlines,
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2) {
                 const int k = k1 + N1 * k2;
-                v[k2] = cmul_hl(v[k2], table_at<T, UNI>(tab, 2 * k), table_at<T, UNI>(tab, 2 * k + 1));
+                v[k2] = cmul_hl(v[k2], table_at<T, UNI, GT>(tab, 2 * k), table_at<T, UNI, GT>(tab, 2 * k + 1));
             }
             dft<N2, -S>(v);
 #pragma unroll
@@ -441,7 +463,7 @@ __device__ __forceinline__ void phaseA_regs(cplx<T> *v, int m, const cplx<T> *__
 }
 
 // PG: elements between two pads of PAD (the innermost group N3 by default).
-template <typename T, int N, int S, int LS, int ES, int PAD = 0, int PG = 0>
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, int PG = 0, bool GT = false>
 __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
                                              const cplx<T> *__restrict__ tw, int tid, int nthr) {
     using Q = Split3<N>;
@@ -497,7 +519,7 @@ __device__ __forceinline__ void line_op_set3(const LineSet<T> &ls, int nlines,
 #pragma unroll
         for (int k3 = 0; k3 < N3; ++k3) {
             const int k = k1 + N1 * (k2 + N2 * k3);
-            v[k3] = cmul_hl(v[k3], tab[2 * k], tab[2 * k + 1]);
+            v[k3] = cmul_hl(v[k3], table_at<T, false, GT>(tab, 2 * k), table_at<T, false, GT>(tab, 2 * k + 1));
         }
         dft<N3, -S>(v);
 #pragma unroll
@@ -562,28 +584,11 @@ template <typename T, int N> constexpr int inner_len() {
 
 // PG: pad group of a three-level line (line_op_set3); two-level lines pad
 // every N2 elements.
-template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false, int PG = 0>
+template <typename T, int N, int S, int LS, int ES, int PAD = 0, bool UNI = false, int PG = 0, bool GT = false>
 __device__ __forceinline__ void line_op_any(const LineSet<T> &ls, int nlines, const cplx<T> *__restrict__ tw,
                                             int tid, int nthr) {
-    if constexpr (three_level<T, N>()) line_op_set3<T, N, S, LS, ES, PAD, PG>(ls, nlines, tw, tid, nthr);
-    else line_op_set<T, N, S, LS, ES, PAD, UNI>(ls, nlines, tw, tid, nthr);
-}
-
-// Global-memory accesses through pointers whose address space the compiler
-// cannot see (field and table pointers read from a solve descriptor in
-// memory).  Left generic, they compile to flat instructions, which count
-// against the LDS wait counter too: every LDS wait of a line operator would
-// then also wait for the prefetch of the next tile or plane.  The pointer
-// must address global memory; values move component-wise (see ld_c).
-#define OPO_GLOBAL __attribute__((address_space(1)))
-template <typename T> __device__ __forceinline__ cplx<T> gld_c(const cplx<T> *p) {
-    const OPO_GLOBAL cplx<T> *q = (const OPO_GLOBAL cplx<T> *)p;
-    return cplx<T>{q->x, q->y};
-}
-template <typename T> __device__ __forceinline__ void gst_c(cplx<T> *p, cplx<T> v) {
-    OPO_GLOBAL cplx<T> *q = (OPO_GLOBAL cplx<T> *)p;
-    q->x = v.x;
-    q->y = v.y;
+    if constexpr (three_level<T, N>()) line_op_set3<T, N, S, LS, ES, PAD, PG, GT>(ls, nlines, tw, tid, nthr);
+    else line_op_set<T, N, S, LS, ES, PAD, UNI, GT>(ls, nlines, tw, tid, nthr);
 }
 
 // Copy of TOT elements by BS threads, element j of thread tid being

@@ -266,7 +266,9 @@ __device__ __forceinline__ void nl_time_tile(const TLine3Args<T> &a, int nx, int
     }
     __syncthreads();
     const LineSet<T> ls{lds + a.first_field * FT, FT, a.tab0, a.tab_stride, 3 - a.first_field};
-    line_op_any<T, NT, +1, 1, TX, L::PAD, false, L::G>(ls, TX, STW ? twl : tw, tid, BS);
+    // the line tables are in global memory (measured: the batched 512-sample
+    // t pass 0.5 % faster than with generic reads; the xy planes were slower)
+    line_op_any<T, NT, +1, 1, TX, L::PAD, false, L::G, true>(ls, TX, STW ? twl : tw, tid, BS);
     if constexpr (TX % 2 == 0) {
         for (int i = 2 * tid; i < TILE; i += 2 * BS) {
             const int t = i / TX, xl = i % TX;
