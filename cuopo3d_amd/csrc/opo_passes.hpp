@@ -384,9 +384,20 @@ __device__ __forceinline__ void nl_time_run(const Src &src, cplx<T> *lds, cplx<T
 template <typename T, int NT>
 __device__ __forceinline__ void stage_time_tables(const cplx<T> *__restrict__ tw, const TLine3Args<T> &a,
                                                   cplx<T> *stab, int tid, int nthr) {
-    for (int i = tid; i < 2 * NT; i += nthr) stab[i] = gld_c(tw + i);
-    for (int f = 0; f < 3 - a.first_field; ++f)
-        for (int i = tid; i < 2 * NT; i += nthr) stab[2 * NT * (1 + f) + i] = gld_c(a.tab0 + (size_t)f * a.tab_stride + i);
+    // the four arrays' loads of an index go out together (one memory
+    // latency per index instead of one per array)
+    const int nf = 3 - a.first_field;
+    for (int i = tid; i < 2 * NT; i += nthr) {
+        cplx<T> v[4];
+        v[0] = gld_c(tw + i);
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+            if (f < nf) v[1 + f] = gld_c(a.tab0 + (size_t)f * a.tab_stride + i);
+        stab[i] = v[0];
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+            if (f < nf) stab[2 * NT * (1 + f) + i] = v[1 + f];
+    }
 }
 
 // Work units of the fused pass of one solve: its tiles.
@@ -501,12 +512,23 @@ template <typename T, int NX>
 __device__ __forceinline__ void stage_xy_tables(const cplx<T> *__restrict__ tw, const cplx<T> *const *tx,
                                                 const cplx<T> *const *ty, int nf, cplx<T> *stab, int tid,
                                                 int nthr) {
-    for (int i = tid; i < 2 * NX; i += nthr) stab[i] = gld_c(tw + i);
-    for (int f = 0; f < nf; ++f)
-        for (int i = tid; i < 2 * NX; i += nthr) {
-            stab[2 * NX * (1 + f) + i] = gld_c(tx[f] + i);
-            stab[2 * NX * (4 + f) + i] = gld_c(ty[f] + i);
-        }
+    // all seven arrays' loads of an index go out together
+    for (int i = tid; i < 2 * NX; i += nthr) {
+        cplx<T> w = gld_c(tw + i), vx[3], vy[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+            if (f < nf) {
+                vx[f] = gld_c(tx[f] + i);
+                vy[f] = gld_c(ty[f] + i);
+            }
+        stab[i] = w;
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+            if (f < nf) {
+                stab[2 * NX * (1 + f) + i] = vx[f];
+                stab[2 * NX * (4 + f) + i] = vy[f];
+            }
+    }
 }
 
 // Work units of the xy pass of one solve: plane t of field f, u = f nt + t.
