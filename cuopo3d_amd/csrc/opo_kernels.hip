@@ -519,10 +519,10 @@ bool xyplane_supported(int nx, int ny, bool fp64) {
 // NX != NY: Q_x on bands of rows, then Q_y on bands of columns, each band
 // in LDS (k_xrows, k_ycols); bands of 32 rows or columns where the other
 // dimension allows, else 16 (every supported length is a multiple of 16).
-// 256-point lines take the fastest square-plane band shapes: FP32 bands of
-// 32 on 512 threads (one 16-point DFT per thread and phase), FP64 bands of
-// 16 (64 KiB of LDS, two workgroups per CU).  512-point lines take bands
-// of 16 (FP32 66 KiB, FP64 131 KiB of LDS).
+// 256- and 512-point lines take the fastest square-plane band shapes:
+// FP32 bands of 32 on 512 threads (256: one 16-point DFT per thread and
+// phase; 512: 131 KiB of LDS, as fast as bands of 16 on 256 threads), FP64
+// bands of 16 (256: 64 KiB of LDS, two workgroups per CU; 512: 131 KiB).
 template <typename T, int N> constexpr int max_band() {
     return (sizeof(T) == 8 && N >= 256) ? 16 : 32;
 }
